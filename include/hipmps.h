@@ -1,0 +1,187 @@
+/*
+ * hipmps.h -- C ABI of libhipmps, the MI355X (gfx950) matrix-product-state circuit
+ * simulator behind the "hip-mps" TNQVM visitor.
+ *
+ * Every entry point replaces one piece of the exatn-mps visitor's ExaTN-backed
+ * arithmetic (citations relative to the tnqvm source tree):
+ *
+ *   hipmps_state_create ..... ExatnMpsVisitor::initialize, |0..0> MPS with bonds 1
+ *                             (tnqvm/visitors/exatn-mps/ExaTnMpsVisitor.cpp:173-345)
+ *   hipmps_apply_1q ......... ExatnMpsVisitor::applyGate, 1-qubit path (:1185-1292)
+ *   hipmps_apply_2q ......... ExatnMpsVisitor::applyTwoQubitGate + truncateSvdTensors
+ *                             (:1387-1730, :2366-2536)
+ *   hipmps_flush ............ the exatn::sync() calls that close a gate (:1289, :1730)
+ *   hipmps_expval_z_parity .. finalize's exp-val-z (:616-643)
+ *   hipmps_norm ............. finalize's "norm" (:604-612)
+ *   hipmps_amplitudes ....... computeWaveFuncSlice with a full bit string (:2588-2675)
+ *   hipmps_wavefunction_slice computeWaveFuncSlice with open legs (-1) (:2588-2675, :794-820)
+ *   hipmps_state_vector ..... finalize's full network evaluation for n < 20 (:593-597)
+ *   hipmps_expval_z_all,
+ *   hipmps_expval_zz ........ no counterpart (per-qubit <Z_i> / <Z_i Z_j> by environments;
+ *                             the ITensor visitor's environment method is the model,
+ *                             tnqvm/visitors/itensor/mps/ITensorMPSVisitor.cpp:173-250)
+ *   hipmps_gate_matrix ...... GateTensorConstructor::getGateTensor + GetGateMatrix<>
+ *                             (tnqvm/visitors/exatn-mps/ExatnUtils.cpp:36-133,
+ *                             tnqvm/base/Gates.hpp:131-339)
+ *   hipmps_batch_* .......... no counterpart: many circuits advanced together
+ *   hipmps_serve_* .......... no counterpart: continuous-batching request loop
+ *
+ * Conventions
+ *   - complex numbers are interleaved (re, im) doubles;
+ *   - gate matrices are row-major; for a 2-qubit gate the FIRST qubit argument is the
+ *     most significant bit of the 4x4 row/column index (tnqvm/base/Gates.hpp:237-246);
+ *   - qubit k is bit k of a state-vector index (ExaTnMpsVisitor.cpp:617-633);
+ *   - every call returns 0 on success or a negative HIPMPS_E_* code; the message of the
+ *     last failure on the calling thread is hipmps_last_error().  Nothing aborts.
+ *
+ * Threading: an engine may be used from several threads for batch/serve calls; a state
+ * handle must be used by one thread at a time.
+ */
+#ifndef HIPMPS_H_
+#define HIPMPS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPMPS_OK 0
+#define HIPMPS_E_ARG -1          /* bad argument (null, range, shape) */
+#define HIPMPS_E_NONADJACENT -2  /* 2-qubit gate on non-neighbouring qubits */
+#define HIPMPS_E_NOMEM -3        /* device allocation / admission failed */
+#define HIPMPS_E_CONVERGENCE -4  /* Jacobi SVD did not converge within its sweep cap */
+#define HIPMPS_E_DEVICE -5       /* HIP runtime error */
+#define HIPMPS_E_NUMERIC -6      /* NaN / Inf met in a tensor */
+#define HIPMPS_E_UNSUPPORTED -7  /* request outside what the build supports */
+
+/* precision */
+#define HIPMPS_FP64 0   /* complex128 everywhere ("plain") */
+#define HIPMPS_MIXED 1  /* complex64 site tensors and contractions, FP64 SVD and norms */
+
+/* partial-norm definition used by the truncation rule (ExaTnMpsVisitor.cpp:2421-2429) */
+#define HIPMPS_PN_SQUARED 0  /* sum |x|^2 (the reference's comment) */
+#define HIPMPS_PN_L2 1       /* sqrt(sum |x|^2) */
+
+/* where singular values go after the split */
+#define HIPMPS_ABSORB_SQRT 0  /* sqrt(S) into both factors (decomposeTensorSVDLR, :1623) */
+#define HIPMPS_ABSORB_RIGHT 1 /* S into the right factor */
+
+typedef struct hipmps_engine_s* hipmps_engine;
+typedef struct hipmps_state_s* hipmps_state;
+
+typedef struct {
+  int device;               /* HIP device ordinal */
+  size_t hbm_budget_bytes;  /* admission budget; 0 = 90% of the device's memory */
+  int jacobi_max_sweeps;    /* 0 = default (40) */
+  double jacobi_tol;        /* 0 = default (relative off-diagonal 1e-13 fp64) */
+} hipmps_engine_opts;
+
+typedef struct {
+  double svd_cutoff;  /* "svd-cutoff"; default DBL_MIN (:257-263) */
+  int max_bond_dim;   /* "max-bond-dim"; <= 0 means INT_MAX-1 (:265-271) */
+  int precision;      /* HIPMPS_FP64 | HIPMPS_MIXED */
+  int renormalize;    /* 0 = reference behaviour (no renormalisation) */
+  int partial_norm;   /* HIPMPS_PN_* */
+  int absorb;         /* HIPMPS_ABSORB_* */
+} hipmps_trunc_opts;
+
+typedef struct {
+  int nq;        /* 1 or 2 */
+  int q0, q1;    /* q1 ignored for nq == 1 */
+  int pad;
+  double m[32];  /* row-major 2x2 (8 doubles) or 4x4 (32 doubles), interleaved */
+} hipmps_gate;
+
+typedef struct {
+  double norm;             /* <psi|psi> after the last flush */
+  double discarded_weight; /* sum of squared discarded singular values */
+  double fidelity_est;     /* prod_g (1 - eps_g), eps_g = discarded / kept+discarded */
+  int max_bond;            /* largest bond dimension reached */
+  int n_2q;                /* 2-qubit gates applied */
+  int n_1q;                /* 1-qubit gates applied (fused or not) */
+  int jacobi_sweeps_max;   /* worst sweep count of one SVD */
+  long long jacobi_sweeps_total;
+  size_t device_bytes;     /* device memory held by the state */
+} hipmps_stats_t;
+
+/* ---- engine ---------------------------------------------------------------------- */
+int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out);
+int hipmps_engine_destroy(hipmps_engine e);
+const char* hipmps_last_error(void);
+const char* hipmps_version(void);
+
+/* gate matrix by name (I H X Y Z S Sdg T Tdg Rx Ry Rz U CNOT CX CZ CY CH CRZ CPhase Swap
+ * iSwap fSim); returns 1 or 2 (qubits) or a negative code for an unknown gate */
+int hipmps_gate_matrix(const char* name, const double* params, int n_params, double* out);
+
+/* ---- one circuit -------------------------------------------------------------------- */
+int hipmps_state_create(hipmps_engine e, int n_qubits, const hipmps_trunc_opts* t,
+                        hipmps_state* out);
+int hipmps_state_destroy(hipmps_state s);
+int hipmps_state_reset(hipmps_state s); /* back to |0..0> */
+int hipmps_apply_1q(hipmps_state s, int q, const double m[8]);
+int hipmps_apply_2q(hipmps_state s, int q0, int q1, const double m[32]); /* |q0-q1| == 1 */
+int hipmps_apply_gates(hipmps_state s, int n, const hipmps_gate* gates);
+int hipmps_flush(hipmps_state s);
+
+int hipmps_bond_dims(hipmps_state s, int* out_n_plus_1);
+int hipmps_stats(hipmps_state s, hipmps_stats_t* out);
+int hipmps_norm(hipmps_state s, double* out);
+int hipmps_expval_z_parity(hipmps_state s, const int* qubits, int k, double* out);
+int hipmps_expval_z_all(hipmps_state s, double* out_n);
+int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out);
+int hipmps_amplitudes(hipmps_state s, const int8_t* bits, int k, double* out_re_im);
+int hipmps_wavefunction_slice(hipmps_state s, const int* bits, double* out_re_im, size_t cap);
+int hipmps_state_vector(hipmps_state s, double* out_re_im, size_t cap); /* n <= 26 */
+/* copy one site tensor (chi_l x 2 x chi_r, complex128) to / from the host */
+int hipmps_get_site(hipmps_state s, int k, double* out, size_t cap);
+int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* in);
+
+/* ---- many circuits advanced together (no counterpart in the reference) ------------- */
+typedef struct {
+  int n_qubits;
+  int n_gates;
+  const hipmps_gate* gates;
+  hipmps_trunc_opts trunc;
+  int n_zz;           /* number of <Z_i Z_j> pairs to report */
+  const int* zz_pairs;
+} hipmps_circuit;
+
+typedef struct {
+  double* expval_z;   /* caller buffer of n_qubits doubles, or NULL */
+  double* expval_zz;  /* caller buffer of n_zz doubles, or NULL */
+  double norm;
+  double discarded_weight;
+  double fidelity_est;
+  int max_bond;
+  int status;
+  double latency_ms;  /* admission to completion */
+} hipmps_result;
+
+int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits,
+                     hipmps_result* results);
+
+/* ---- serving: continuous batching (no counterpart in the reference) ------------------ */
+int hipmps_serve_submit(hipmps_engine e, const hipmps_circuit* c, hipmps_result* result,
+                        uint64_t* ticket);
+int hipmps_serve_step(hipmps_engine e, int* n_done);   /* advance every in-flight circuit */
+int hipmps_serve_poll(hipmps_engine e, uint64_t ticket, int* done);
+int hipmps_serve_pending(hipmps_engine e, int* queued, int* running);
+
+/* ---- measurement hooks for the bench ------------------------------------------------ */
+typedef struct {
+  double ms_theta, ms_jacobi, ms_split, ms_other;
+  long long jacobi_launches;
+  long long jacobi_pairs_rotated;
+  long long jacobi_pairs_visited;
+  double jacobi_flops; /* flops executed by the Jacobi block-rotation kernel */
+} hipmps_profile_t;
+int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset);
+int hipmps_engine_set_timing(hipmps_engine e, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIPMPS_H_ */

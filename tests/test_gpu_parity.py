@@ -1,0 +1,219 @@
+"""GPU parity of libhipmps against the CPU FP64 oracle, through the C ABI.
+
+Bars: FP64 path <Z> / amplitudes within 1e-10 of the dense state vector (no
+truncation) and of the oracle MPS restatement (with truncation); mixed path <Z>
+within 1e-5.  Singular values against LAPACK spectra in tests/golden.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tnqvm_amd import circuits as Cc
+from tnqvm_amd import mps
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL64 = 1e-10
+TOLMIX = 1e-5
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = mps.Engine()
+    yield e
+    e.close()
+
+
+def g(name, *qubits, params=()):
+    return {"name": name, "qubits": tuple(qubits), "params": tuple(params)}
+
+
+def rand_adjacent(n, n_gates, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_gates):
+        if n > 1 and rng.random() < 0.5:
+            q = int(rng.integers(0, n - 1))
+            pair = (q, q + 1) if rng.random() < 0.5 else (q + 1, q)
+            out.append({"name": "unitary", "qubits": pair, "matrix": Cc.haar_unitary(rng, 4)})
+        else:
+            out.append({"name": "unitary", "qubits": (int(rng.integers(0, n)),), "matrix": Cc.haar_unitary(rng, 2)})
+    return out
+
+
+def test_kats(eng):
+    say("kats")
+    # Bell, both operand orders
+    for ops in ([g("H", 0), g("CNOT", 0, 1)], [g("H", 1), g("CNOT", 1, 0)]):
+        s = mps.MpsState(eng, 4).run(ops)
+        p = np.abs(s.state_vector()) ** 2
+        assert abs(p[0] - 0.5) < 1e-14 and abs(p[3] - 0.5) < 1e-14
+    # iSwap: X(q0); iSwap(q0,q1) -> |010>
+    s = mps.MpsState(eng, 3).run([g("X", 0), g("iSwap", 0, 1)])
+    assert abs(abs(s.state_vector()[2]) ** 2 - 1) < 1e-14
+    # X chain with CNOT ladder -> 1111
+    s = mps.MpsState(eng, 4).run([g("X", 0), g("CNOT", 0, 1), g("CNOT", 1, 2), g("CNOT", 2, 3)])
+    assert abs(s.amplitudes([[1, 1, 1, 1]])[0] - 1) < 1e-14
+    # Rx: <Z> = cos theta
+    for th in np.linspace(-np.pi, np.pi, 7):
+        s = mps.MpsState(eng, 1).run([g("Rx", 0, params=(th,))])
+        assert abs(s.expval_z_parity([0]) - np.cos(th)) < 1e-14
+
+
+def test_deuteron_table(eng):
+    say("deuteron")
+    import json
+    kat = json.load(open(os.path.join(GOLD, "kat.json")))
+    for t, ref in zip(kat["deuteron_angles"], kat["deuteron_x0x1"]):
+        s = mps.MpsState(eng, 2).run([g("X", 0), g("Ry", 1, params=(t,)), g("CNOT", 1, 0), g("H", 0), g("H", 1)])
+        assert abs(s.expval_z_parity([0, 1]) - ref) < 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8])
+def test_random_adjacent_vs_dense(eng, n):
+    say("random_adjacent", n)
+    gates = rand_adjacent(n, 60, seed=100 + n)
+    ref = O.StateVector(n).run(gates)
+    s = mps.MpsState(eng, n).run(gates)
+    assert np.abs(s.state_vector() - ref.psi).max() < 1e-12
+    ez = s.expval_z_all()
+    for q in range(n):
+        assert abs(ez[q] - ref.expval_z_parity([q])) < 1e-12
+    assert abs(s.norm() - 1.0) < 1e-12
+
+
+def test_u3_cnot_line_fp64(eng):
+    say("u3_cnot_line fp64")
+    gates = Cc.u3_cnot_line()
+    ref = O.StateVector(12).run(gates)
+    s = mps.MpsState(eng, 12).run(gates)
+    assert list(s.bonds()) == [1, 2, 4, 8, 16, 32, 64, 32, 16, 8, 4, 2, 1]
+    ez = s.expval_z_all()
+    for q in range(12):
+        assert abs(ez[q] - ref.expval_z_parity([q])) < TOL64
+        assert abs(s.expval_z_parity([q]) - ref.expval_z_parity([q])) < TOL64
+    bits = Cc.random_bitstrings(12, 16, 5)
+    assert np.abs(s.amplitudes(bits) - ref.amplitudes(bits)).max() < TOL64
+    assert abs(s.norm() - ref.norm()) < TOL64
+    pairs = [(q, q + 1) for q in range(11)] + [(0, 11), (3, 7)]
+    zz = s.expval_zz(pairs)
+    for (a, b), v in zip(pairs, zz):
+        assert abs(v - ref.expval_z_parity([a, b])) < TOL64
+
+
+def test_u3_cnot_line_mixed(eng):
+    say("u3_cnot_line mixed")
+    gates = Cc.u3_cnot_line()
+    ref = O.StateVector(12).run(gates)
+    s = mps.MpsState(eng, 12, precision="mixed").run(gates)
+    ez = s.expval_z_all()
+    for q in range(12):
+        assert abs(ez[q] - ref.expval_z_parity([q])) < TOLMIX
+
+
+def test_slice(eng):
+    say("slice")
+    gates = rand_adjacent(6, 40, seed=11)
+    ref = O.StateVector(6).run(gates)
+    s = mps.MpsState(eng, 6).run(gates)
+    bits = [1, -1, 0, -1, -1, 1]
+    sl = s.slice(bits)
+    exp = []
+    for x in range(8):
+        b = [1, x & 1, 0, (x >> 1) & 1, (x >> 2) & 1, 1]
+        exp.append(ref.psi[sum(bb << i for i, bb in enumerate(b))])
+    assert np.abs(sl - np.array(exp)).max() < 1e-13
+
+
+@pytest.mark.parametrize("cut,maxb", [(0.5, 0), (1e-3, 4), (1e-8, 3), (0.0, 2)])
+def test_truncation_vs_oracle_mps(eng, cut, maxb):
+    say("truncation", cut, maxb)
+    gates = rand_adjacent(7, 80, seed=7)
+    m = O.Mps(7, cutoff=cut, max_bond=maxb).run(gates)
+    s = mps.MpsState(eng, 7, cutoff=cut, max_bond=maxb).run(gates)
+    assert list(s.bonds()) == list(m.bonds())
+    assert np.abs(s.state_vector() - m.state_vector()).max() < TOL64
+    assert abs(s.stats()["discarded_weight"] - m.discarded()) < 1e-10
+
+
+def test_svd_golden_spectra(eng):
+    """theta' = golden matrix through a 4-site MPS with an identity gate on (1,2);
+    the new left site's bond-column norms^2 are the singular values (sqrt split)."""
+    z = np.load(os.path.join(GOLD, "svd_spectra.npz"))
+    for key in ["gauss_16x16", "gauss_32x32", "gauss_64x64", "graded_64x64", "rank3_16x16", "gauss_96x128",
+                "gauss_128x96"]:
+        say("svd", key)
+        a, sref = z["A_" + key], z["S_" + key]
+        m, n = a.shape
+        cl, cr = m // 2, n // 2
+        s = mps.MpsState(eng, 4)
+        s.set_site(0, np.ones((1, 2, cl)) / np.sqrt(2 * cl))
+        s.set_site(1, a.reshape(cl, 2, n))          # chi_M = n
+        s.set_site(2, np.eye(n).reshape(n, 2, cr))  # theta = A1 A2 = a
+        s.set_site(3, np.ones((cr, 2, 1)) / np.sqrt(2 * cr))
+        s.apply({"name": "unitary", "qubits": (1, 2), "matrix": np.eye(4)})
+        s.flush()
+        A1 = s.get_site(1).reshape(m, -1)
+        A2 = s.get_site(2).reshape(-1, n)
+        sig = (np.abs(A1) ** 2).sum(0)
+        kk = min(len(sig), len(sref))
+        assert np.allclose(sig[:kk], sref[:kk], atol=1e-12 * sref[0]), key
+        assert np.abs(A1 @ A2 - a).max() < 1e-12 * sref[0], key
+
+
+def test_batch_matches_single(eng):
+    say("batch")
+    jobs = []
+    for k, n in enumerate([5, 8, 12]):
+        gates = rand_adjacent(n, 50, seed=200 + k)
+        jobs.append({"n": n, "gates": gates, "zz": [(0, 1), (1, n - 1)]})
+    res = eng.run_batch(jobs)
+    for j, r in zip(jobs, res):
+        ref = O.StateVector(j["n"]).run(j["gates"])
+        for q in range(j["n"]):
+            assert abs(r["expval_z"][q] - ref.expval_z_parity([q])) < TOL64
+        for (a, b), v in zip(j["zz"], r["expval_zz"]):
+            assert abs(v - ref.expval_z_parity([a, b])) < TOL64
+        assert abs(r["norm"] - 1.0) < 1e-10
+
+
+def test_serving_alone_equals_mixed(eng):
+    say("serving")
+    srv = mps.Server(eng)
+    jobs = [(n, rand_adjacent(n, 40 + 5 * k, seed=300 + k)) for k, n in enumerate([4, 9, 6, 11])]
+    alone = []
+    for n, gates in jobs:
+        t = srv.submit(n, gates)
+        r = None
+        while r is None:
+            srv.step()
+            r = srv.poll(t)
+        alone.append(r)
+    # resubmit all at once, staggered arrival
+    tickets = []
+    for k, (n, gates) in enumerate(jobs):
+        tickets.append(srv.submit(n, gates))
+        srv.step()
+    results = {}
+    while len(results) < len(tickets):
+        srv.step()
+        for t in tickets:
+            if t not in results:
+                r = srv.poll(t)
+                if r is not None:
+                    results[t] = r
+    for (n, gates), t, a in zip(jobs, tickets, alone):
+        ref = O.StateVector(n).run(gates)
+        ez = results[t]["expval_z"]
+        for q in range(n):
+            assert abs(ez[q] - ref.expval_z_parity([q])) < TOL64
+        # the same circuit alone and in the mix gives the same numbers
+        assert np.abs(ez - a["expval_z"]).max() < 1e-13

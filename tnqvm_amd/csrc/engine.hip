@@ -1,0 +1,1466 @@
+// engine.hip -- host side of libhipmps: engine, MPS state, wave scheduler, batched
+// gate application, measurement sweeps and the continuous-batching serving loop.
+//
+// Reference mapping (tnqvm/visitors/exatn-mps/ExaTnMpsVisitor.cpp):
+//   state create / reset ....... initialize (:173-345): |0..0>, all bonds 1
+//   apply_1q ................... applyGate (:1185-1292)
+//   apply_2q + wave execution .. applyTwoQubitGate (:1387-1730) + truncateSvdTensors
+//                                (:2366-2536)
+//   norm / expval_z_parity ..... finalize (:593-643)
+//   amplitudes / slice ......... computeWaveFuncSlice (:2588-2675) and the slice
+//                                normalisation (:794-820, done by the caller)
+//
+// Execution model.  Every 2-qubit gate becomes a task of a "wave": a set of gates on
+// pairwise disjoint site pairs, possibly from many circuits.  One wave runs as a
+// fixed sequence of batched launches on the engine stream:
+//   k_theta -> k_jacobi_prep -> (k_jacobi_round x rounds) x sweeps -> k_rownorm ->
+//   k_select -> k_gather + k_gemm
+// so the device sees all independent SVDs of the wave at once.  1-qubit gates are
+// fused into the next 2-qubit gate on the same qubit (U_2q . (U_a x U_b)); whatever is
+// left over is applied by k_apply_1q when the state is flushed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hipmps.h"
+#include "kernels.h"
+
+using cd = std::complex<double>;
+using namespace hipmps;
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string& msg) {
+  t_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess) return fail(HIPMPS_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define RC_TRY(expr)          \
+  do {                        \
+    int _rc = (expr);         \
+    if (_rc != 0) return _rc; \
+  } while (0)
+
+constexpr size_t ALIGN = 256;
+inline size_t align_up(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t nb = std::max(need, (size_t)1 << 20);
+    nb = std::max(nb, need + need / 4);
+    if (hipMalloc(&p, nb) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(HIPMPS_E_NOMEM, "device workspace allocation of " + std::to_string(nb) + " bytes failed");
+    }
+    bytes = nb;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return 0;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    size_t nb = std::max(need, (size_t)1 << 16);
+    if (hipHostMalloc(&p, nb, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      bytes = 0;
+      (void)hipGetLastError();
+      return fail(HIPMPS_E_NOMEM, "pinned host allocation failed");
+    }
+    bytes = nb;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// 2x2 / 4x4 helpers (row-major)
+void mat_mul4(const cd* A, const cd* B, cd* C) {
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      cd s = 0;
+      for (int k = 0; k < 4; ++k) s += A[i * 4 + k] * B[k * 4 + j];
+      C[i * 4 + j] = s;
+    }
+}
+void mat_mul2(const cd* A, const cd* B, cd* C) {
+  cd t[4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) t[i * 2 + j] = A[i * 2] * B[j] + A[i * 2 + 1] * B[2 + j];
+  for (int i = 0; i < 4; ++i) C[i] = t[i];
+}
+void kron2(const cd* A, const cd* B, cd* K) {  // A (x) B, A on the most significant bit
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int a2 = 0; a2 < 2; ++a2)
+        for (int b2 = 0; b2 < 2; ++b2) K[(a * 2 + b) * 4 + (a2 * 2 + b2)] = A[a * 2 + a2] * B[b * 2 + b2];
+}
+
+}  // namespace
+
+// ======================================================================================
+// engine / state
+// ======================================================================================
+struct Site {
+  void* p = nullptr;
+  size_t cap = 0;  // elements
+};
+
+struct hipmps_state_s;
+
+struct PendingCircuit;
+
+struct hipmps_engine_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t used = 0;  // bytes of site storage held by states
+  int max_sweeps = 40;
+  double tol = 1e-13;
+  int inner_sweeps = 10;
+  DevBuf ws;     // per-wave arena
+  DevBuf desc;   // task descriptors
+  DevBuf small;  // counters / K / stats
+  DevBuf env;    // environments for measurements
+  HostBuf hdesc;
+  HostBuf hsmall;
+  std::mutex mu;
+  bool timing = false;
+  hipmps_profile_t prof{};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // serving
+  uint64_t next_ticket = 1;
+  std::deque<PendingCircuit*> queue;
+  std::vector<PendingCircuit*> running;
+  std::map<uint64_t, PendingCircuit*> by_ticket;
+};
+
+struct hipmps_state_s {
+  hipmps_engine e = nullptr;
+  int n = 0;
+  hipmps_trunc_opts t{};
+  int fp32 = 0;
+  size_t es = 16;
+  std::vector<int> chi;
+  std::vector<Site> sites;
+  std::vector<std::array<cd, 4>> pend;
+  std::vector<char> has_pend;
+  struct Q2 {
+    int i;
+    cd G[16];
+  };
+  std::vector<Q2> wave;
+  std::vector<char> busy;
+  double discarded = 0.0;
+  double fid = 1.0;
+  int maxb = 1;
+  int n2q = 0, n1q = 0;
+  int sweeps_max = 0;
+  long long sweeps_total = 0;
+  size_t bytes = 0;
+  int status = 0;
+  std::vector<void*> deferred;
+};
+
+struct PendingCircuit {
+  uint64_t ticket = 0;
+  hipmps_circuit c{};
+  std::vector<hipmps_gate> gates;
+  std::vector<int> zz;
+  hipmps_result* res = nullptr;
+  hipmps_state_s* st = nullptr;
+  size_t cursor = 0;
+  size_t footprint = 0;
+  bool done = false;
+  std::chrono::steady_clock::time_point t0;
+};
+
+namespace {
+
+int set_device(hipmps_engine e) {
+  HIP_TRY(hipSetDevice(e->device));
+  return 0;
+}
+
+// bond capacity bound: min(2^b, 2^(n-b), chi_max)
+size_t bond_bound(int n, int b, int maxb) {
+  int e = std::min(b, n - b);
+  size_t v = e >= 30 ? ((size_t)1 << 30) : ((size_t)1 << e);
+  return std::min(v, (size_t)maxb);
+}
+
+size_t projected_bytes(int n, int maxb, int fp32) {
+  size_t es = fp32 ? 8 : 16, tot = 0;
+  for (int k = 0; k < n; ++k) tot += bond_bound(n, k, maxb) * 2 * bond_bound(n, k + 1, maxb) * es;
+  return tot;
+}
+
+int ensure_site(hipmps_state_s* s, int k, size_t elems, bool keep_old_for_read, void** old_out) {
+  Site& S = s->sites[k];
+  if (old_out) *old_out = S.p;
+  if (elems <= S.cap) return 0;
+  size_t ncap = std::max(elems, S.cap * 2);
+  size_t nbytes = ncap * s->es;
+  hipmps_engine e = s->e;
+  if (e->used + nbytes > e->budget)
+    return fail(HIPMPS_E_NOMEM, "HBM budget exceeded growing site " + std::to_string(k));
+  void* p = nullptr;
+  if (hipMalloc(&p, nbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(HIPMPS_E_NOMEM, "device allocation for site " + std::to_string(k) + " failed");
+  }
+  if (S.p) {
+    if (keep_old_for_read) s->deferred.push_back(S.p);
+    else (void)hipFree(S.p);
+    e->used -= S.cap * s->es;
+    s->bytes -= S.cap * s->es;
+  }
+  S.p = p;
+  S.cap = ncap;
+  e->used += nbytes;
+  s->bytes += nbytes;
+  return 0;
+}
+
+void release_deferred(hipmps_state_s* s) {
+  for (void* p : s->deferred) (void)hipFree(p);
+  s->deferred.clear();
+}
+
+int upload_site_host(hipmps_state_s* s, int k, const cd* data, size_t n) {
+  hipmps_engine e = s->e;
+  RC_TRY(e->hsmall.ensure(n * 16));
+  if (s->fp32) {
+    float* h = (float*)e->hsmall.p;
+    for (size_t i = 0; i < n; ++i) {
+      h[2 * i] = (float)data[i].real();
+      h[2 * i + 1] = (float)data[i].imag();
+    }
+    HIP_TRY(hipMemcpyAsync(s->sites[k].p, h, n * 8, hipMemcpyHostToDevice, e->stream));
+  } else {
+    memcpy(e->hsmall.p, data, n * 16);
+    HIP_TRY(hipMemcpyAsync(s->sites[k].p, e->hsmall.p, n * 16, hipMemcpyHostToDevice, e->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+int state_init_product(hipmps_state_s* s) {
+  // |0..0>: every site 1 x 2 x 1 with body {1, 0} (ExaTnMpsVisitor.cpp:46, :323)
+  s->chi.assign(s->n + 1, 1);
+  const cd body[2] = {1.0, 0.0};
+  for (int k = 0; k < s->n; ++k) {
+    RC_TRY(ensure_site(s, k, 2, false, nullptr));
+    RC_TRY(upload_site_host(s, k, body, 2));
+  }
+  s->pend.assign(s->n, {cd(1), cd(0), cd(0), cd(1)});
+  s->has_pend.assign(s->n, 0);
+  s->busy.assign(s->n, 0);
+  s->wave.clear();
+  s->discarded = 0.0;
+  s->fid = 1.0;
+  s->maxb = 1;
+  s->n2q = s->n1q = 0;
+  s->sweeps_max = 0;
+  s->sweeps_total = 0;
+  s->status = 0;
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// wave execution (many states, many bonds)
+// ------------------------------------------------------------------------------------
+struct WaveItem {
+  hipmps_state_s* s;
+  int i;          // left site
+  const cd* G;    // 16 entries, local order (a = left-site bit)
+  // derived
+  int cl, cm, cr, m, n, r, c, r_pad, kub, transposed, fp32;
+  void* old_i;
+  void* old_j;
+  size_t off_theta, off_y, off_vec;
+};
+
+int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
+  if (items.empty()) return 0;
+  const hipStream_t st = E->stream;
+  // order: fp64 before fp32, non-transposed before transposed (contiguous launch groups)
+  for (auto& it : items) {
+    hipmps_state_s* s = it.s;
+    it.cl = s->chi[it.i];
+    it.cm = s->chi[it.i + 1];
+    it.cr = s->chi[it.i + 2];
+    it.m = 2 * it.cl;
+    it.n = 2 * it.cr;
+    it.r = std::min(it.m, it.n);
+    it.c = std::max(it.m, it.n);
+    it.transposed = it.m > it.n;
+    it.r_pad = (it.r + JP - 1) / JP * JP;
+    int maxb = s->t.max_bond_dim > 0 ? s->t.max_bond_dim : INT_MAX - 1;
+    it.kub = std::min(it.r, maxb);
+    it.fp32 = s->fp32;
+  }
+  std::stable_sort(items.begin(), items.end(), [](const WaveItem& a, const WaveItem& b) {
+    if (a.fp32 != b.fp32) return a.fp32 < b.fp32;
+    return a.transposed < b.transposed;
+  });
+  const int N = (int)items.size();
+  for (auto& it : items)
+    if (it.r > 4096) return fail(HIPMPS_E_UNSUPPORTED, "bond dimension above 2048 is not supported");
+  // site capacity (new sites are written after theta has read the old ones)
+  for (auto& it : items) {
+    RC_TRY(ensure_site(it.s, it.i, (size_t)it.cl * 2 * it.kub, true, &it.old_i));
+    RC_TRY(ensure_site(it.s, it.i + 1, (size_t)it.kub * 2 * it.cr, true, &it.old_j));
+  }
+  // arena
+  size_t off = 0;
+  for (auto& it : items) {
+    it.off_theta = off;
+    off = align_up(off + (size_t)it.m * it.n * 16);
+    it.off_y = off;
+    off = align_up(off + (size_t)it.r_pad * (it.c + it.r) * 16);
+    it.off_vec = off;
+    off = align_up(off + (size_t)it.r_pad * (8 * 4 + 4));
+  }
+  RC_TRY(E->ws.ensure(off));
+  char* ws = (char*)E->ws.p;
+  // small: counts (2N ints), K (N ints), wstats (4N doubles)
+  size_t small_bytes = align_up(3 * N * sizeof(int)) + 4 * N * sizeof(double);
+  RC_TRY(E->small.ensure(small_bytes));
+  int* d_counts = (int*)E->small.p;
+  int* d_K = d_counts + 2 * N;
+  double* d_w = (double*)((char*)E->small.p + align_up(3 * N * sizeof(int)));
+  // descriptors
+  size_t desc_bytes = align_up(N * sizeof(ThetaTask)) + align_up(N * sizeof(SplitTask)) +
+                      align_up(N * sizeof(JacTask));
+  RC_TRY(E->desc.ensure(desc_bytes));
+  // host staging: descriptors | counts (2N ints) | active Jacobi list / results
+  const size_t stage_bytes = desc_bytes + align_up(2 * N * sizeof(int)) + N * sizeof(JacTask) + 4 * N * sizeof(double) + 4096;
+  RC_TRY(E->hdesc.ensure(stage_bytes));
+  char* hd = (char*)E->hdesc.p;
+  ThetaTask* hth = (ThetaTask*)hd;
+  SplitTask* hsp = (SplitTask*)(hd + align_up(N * sizeof(ThetaTask)));
+  JacTask* hjt = (JacTask*)((char*)hsp + align_up(N * sizeof(SplitTask)));
+  char* dd = (char*)E->desc.p;
+  ThetaTask* dth = (ThetaTask*)dd;
+  SplitTask* dsp = (SplitTask*)(dd + align_up(N * sizeof(ThetaTask)));
+  JacTask* djt = (JacTask*)((char*)dsp + align_up(N * sizeof(SplitTask)));
+
+  // group boundaries
+  int n64 = 0;
+  while (n64 < N && !items[n64].fp32) ++n64;
+  int max_r = 0, max_c = 0, max_rpad = 0, max_k = 0;
+  int tile_acc[2] = {0, 0};
+  for (int q = 0; q < N; ++q) {
+    auto& it = items[q];
+    const int grp = it.fp32;
+    ThetaTask& T = hth[q];
+    memset(&T, 0, sizeof(T));
+    T.Ai = it.old_i;
+    T.Aj = it.old_j;
+    T.theta = (double2*)(ws + it.off_theta);
+    T.cl = it.cl;
+    T.cm = it.cm;
+    T.cr = it.cr;
+    T.tiles_l = (it.cl + 15) / 16;
+    T.tiles_r = (it.cr + 31) / 32;
+    T.tile_begin = tile_acc[grp];
+    tile_acc[grp] += T.tiles_l * T.tiles_r;
+    for (int k = 0; k < 16; ++k) T.G[k] = make_double2(it.G[k].real(), it.G[k].imag());
+
+    SplitTask& S = hsp[q];
+    memset(&S, 0, sizeof(S));
+    S.theta = T.theta;
+    S.Y = (double2*)(ws + it.off_y);
+    S.m = it.m;
+    S.n = it.n;
+    S.r = it.r;
+    S.c = it.c;
+    S.ldy = it.c + it.r;
+    S.transposed = it.transposed;
+    char* v = ws + it.off_vec;
+    S.rownorm = (double*)v;
+    S.sig = (double*)(v + 8 * it.r_pad);
+    S.scale_y = (double*)(v + 16 * it.r_pad);
+    S.scale_w = (double*)(v + 24 * it.r_pad);
+    S.perm = (int*)(v + 32 * it.r_pad);
+    S.K = d_K + q;
+    S.wstats = d_w + 4 * q;
+    S.Ai = it.s->sites[it.i].p;
+    S.Aj = it.s->sites[it.i + 1].p;
+    S.cl = it.cl;
+    S.cr = it.cr;
+    const hipmps_trunc_opts& to = it.s->t;
+    S.cutoff = to.svd_cutoff;
+    S.max_bond = to.max_bond_dim > 0 ? to.max_bond_dim : INT_MAX - 1;
+    S.pn_l2 = to.partial_norm == HIPMPS_PN_L2;
+    S.absorb = to.absorb == HIPMPS_ABSORB_RIGHT ? 1 : 0;
+    S.renorm = to.renormalize ? 1 : 0;
+    S.site_fp32 = it.fp32;
+
+    JacTask& J = hjt[q];
+    J.Y = (double2*)S.Y;
+    J.ldy = S.ldy;
+    J.r = it.r;
+    J.r_pad = it.r_pad;
+    J.c = it.c;
+    J.c_tot = it.c + it.r;
+    J.nb = it.r_pad / JB;
+    J.counter = d_counts + 2 * q;
+    J.visited = d_counts + 2 * q + 1;
+
+    max_r = std::max(max_r, it.r);
+    max_rpad = std::max(max_rpad, it.r_pad);
+    max_c = std::max(max_c, it.c + it.r);
+    max_k = std::max(max_k, it.kub);
+  }
+  HIP_TRY(hipMemcpyAsync(dd, hd, desc_bytes, hipMemcpyHostToDevice, st));
+
+  if (E->timing) HIP_TRY(hipEventRecord(E->ev[0], st));
+  // ---- theta ----
+  if (n64 > 0) HIP_TRY(launch_theta(dth, n64, tile_acc[0], 0, st));
+  if (N - n64 > 0) HIP_TRY(launch_theta(dth + n64, N - n64, tile_acc[1], 1, st));
+  HIP_TRY(launch_jacobi_prep(dsp, djt, N, max_rpad, max_c, st));
+  if (E->timing) HIP_TRY(hipEventRecord(E->ev[1], st));
+
+  // ---- Jacobi sweeps ----
+  std::vector<int> active(N);
+  for (int q = 0; q < N; ++q) active[q] = q;
+  std::vector<int> sweeps_used(N, E->max_sweeps);
+  int* hcounts = (int*)((char*)E->hdesc.p + desc_bytes);
+  bool failed_conv = false;
+  for (int sweep = 0; sweep < E->max_sweeps && !active.empty(); ++sweep) {
+    // active task list with pair prefix
+    int nact = (int)active.size();
+    int pairs = 0, max_rounds = 0;
+    JacTask* hja = (JacTask*)((char*)E->hdesc.p + desc_bytes + align_up(2 * N * sizeof(int)));
+    for (int a = 0; a < nact; ++a) {
+      hja[a] = hjt[active[a]];
+      hja[a].pair_begin = pairs;
+      pairs += hja[a].nb / 2;
+      max_rounds = std::max(max_rounds, hja[a].nb - 1);
+    }
+    // reuse the JacTask region of the device descriptor block for the active list
+    HIP_TRY(hipMemcpyAsync(djt, hja, nact * sizeof(JacTask), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * N * sizeof(int), st));
+    for (int round = 0; round < max_rounds; ++round)
+      HIP_TRY(launch_jacobi_round(djt, nact, pairs, round, E->tol, E->inner_sweeps, st));
+    E->prof.jacobi_launches += max_rounds;
+    HIP_TRY(hipMemcpyAsync(hcounts, d_counts, 2 * N * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int> next;
+    for (int a = 0; a < nact; ++a) {
+      int q = active[a];
+      long long rot = hcounts[2 * q], vis = hcounts[2 * q + 1];
+      E->prof.jacobi_pairs_rotated += rot;
+      E->prof.jacobi_pairs_visited += vis;
+      const double c = items[q].c, ct = items[q].c + items[q].r;
+      E->prof.jacobi_flops += 8.0 * JP * JP * (c * (double)vis + ct * (double)rot);
+      if (rot == 0) sweeps_used[q] = sweep + 1;
+      else next.push_back(q);
+    }
+    active.swap(next);
+  }
+  if (!active.empty()) failed_conv = true;
+  // restore the full JacTask array (not needed further, kept for clarity of ownership)
+  if (E->timing) HIP_TRY(hipEventRecord(E->ev[2], st));
+
+  // ---- singular values, truncation, factors ----
+  HIP_TRY(launch_rownorm(dsp, N, max_r, st));
+  HIP_TRY(launch_select(dsp, N, st));
+  {
+    int max_len64 = 0, max_len32 = 0, mk64 = 0, mk32 = 0;
+    for (int q = 0; q < N; ++q) {
+      int len = std::max(items[q].m, items[q].n);
+      if (items[q].fp32) { max_len32 = std::max(max_len32, len); mk32 = std::max(mk32, items[q].kub); }
+      else { max_len64 = std::max(max_len64, len); mk64 = std::max(mk64, items[q].kub); }
+    }
+    if (n64 > 0) HIP_TRY(launch_gather(dsp, n64, mk64, max_len64, 0, st));
+    if (N - n64 > 0) HIP_TRY(launch_gather(dsp + n64, N - n64, mk32, max_len32, 1, st));
+  }
+  // results
+  int* hK = hcounts;  // reuse staging
+  double* hw = (double*)((char*)E->hdesc.p + desc_bytes + align_up(2 * N * sizeof(int)));
+  HIP_TRY(hipMemcpyAsync(hK, d_K, N * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hw, d_w, 4 * N * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (E->timing) HIP_TRY(hipEventRecord(E->ev[3], st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (E->timing) {
+    float a = 0, b = 0, c = 0;
+    (void)hipEventElapsedTime(&a, E->ev[0], E->ev[1]);
+    (void)hipEventElapsedTime(&b, E->ev[1], E->ev[2]);
+    (void)hipEventElapsedTime(&c, E->ev[2], E->ev[3]);
+    E->prof.ms_theta += a;
+    E->prof.ms_jacobi += b;
+    E->prof.ms_split += c;
+  }
+  for (int q = 0; q < N; ++q) {
+    auto& it = items[q];
+    hipmps_state_s* s = it.s;
+    int K = hK[q];
+    s->chi[it.i + 1] = K;
+    s->maxb = std::max(s->maxb, K);
+    double disc = hw[4 * q], kept = hw[4 * q + 1];
+    s->discarded += disc;
+    if (kept + disc > 0) s->fid *= kept / (kept + disc);
+    s->n2q++;
+    s->sweeps_max = std::max(s->sweeps_max, sweeps_used[q]);
+    s->sweeps_total += sweeps_used[q];
+    if (failed_conv && sweeps_used[q] >= E->max_sweeps) s->status = HIPMPS_E_CONVERGENCE;
+  }
+  for (auto& it : items) release_deferred(it.s);
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// per-state gate queue
+// ------------------------------------------------------------------------------------
+int state_run_wave(hipmps_state_s* s) {
+  if (s->wave.empty()) return 0;
+  std::vector<WaveItem> items;
+  items.reserve(s->wave.size());
+  for (auto& q : s->wave) {
+    WaveItem it{};
+    it.s = s;
+    it.i = q.i;
+    it.G = q.G;
+    items.push_back(it);
+  }
+  int rc = run_wave(s->e, items);
+  s->wave.clear();
+  std::fill(s->busy.begin(), s->busy.end(), 0);
+  return rc;
+}
+
+// Queue a 2-qubit gate; returns 1 if the gate conflicts with the open wave (caller runs
+// the wave and retries), 0 when queued.
+int queue_2q(hipmps_state_s* s, int q0, int q1, const cd* M, bool* conflict) {
+  const int i = std::min(q0, q1);
+  *conflict = false;
+  if (s->busy[i] || s->busy[i + 1]) {
+    *conflict = true;
+    return 0;
+  }
+  // fuse pending 1-qubit gates: G' = M . (P_q0 (x) P_q1), q0 on the most significant bit
+  cd P0[4] = {1, 0, 0, 1}, P1[4] = {1, 0, 0, 1};
+  if (s->has_pend[q0]) std::copy(s->pend[q0].begin(), s->pend[q0].end(), P0);
+  if (s->has_pend[q1]) std::copy(s->pend[q1].begin(), s->pend[q1].end(), P1);
+  cd Kp[16], Gf[16];
+  kron2(P0, P1, Kp);
+  mat_mul4(M, Kp, Gf);
+  s->has_pend[q0] = s->has_pend[q1] = 0;
+  s->pend[q0] = {cd(1), cd(0), cd(0), cd(1)};
+  s->pend[q1] = {cd(1), cd(0), cd(0), cd(1)};
+  hipmps_state_s::Q2 g;
+  g.i = i;
+  if (q0 == i) {
+    std::copy(Gf, Gf + 16, g.G);
+  } else {  // first operand is the right site: swap the two bits of row and column index
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int a2 = 0; a2 < 2; ++a2)
+          for (int b2 = 0; b2 < 2; ++b2) g.G[(a * 2 + b) * 4 + (a2 * 2 + b2)] = Gf[(b * 2 + a) * 4 + (b2 * 2 + a2)];
+  }
+  s->wave.push_back(g);
+  s->busy[i] = s->busy[i + 1] = 1;
+  return 0;
+}
+
+void queue_1q(hipmps_state_s* s, int q, const cd* M) {
+  cd P[4];
+  std::copy(s->pend[q].begin(), s->pend[q].end(), P);
+  cd R[4];
+  mat_mul2(M, P, R);  // later gate on the left
+  for (int k = 0; k < 4; ++k) s->pend[q][k] = R[k];
+  s->has_pend[q] = 1;
+  s->n1q++;
+}
+
+int flush_pending_1q(hipmps_state_s* s) {
+  for (int q = 0; q < s->n; ++q) {
+    if (!s->has_pend[q]) continue;
+    double2 m[4];
+    for (int k = 0; k < 4; ++k) m[k] = make_double2(s->pend[q][k].real(), s->pend[q][k].imag());
+    HIP_TRY(launch_apply_1q(s->sites[q].p, s->fp32, s->chi[q], s->chi[q + 1], m, s->e->stream));
+    s->has_pend[q] = 0;
+    s->pend[q] = {cd(1), cd(0), cd(0), cd(1)};
+  }
+  return 0;
+}
+
+int state_flush(hipmps_state_s* s) {
+  RC_TRY(set_device(s->e));
+  RC_TRY(state_run_wave(s));
+  RC_TRY(flush_pending_1q(s));
+  HIP_TRY(hipStreamSynchronize(s->e->stream));
+  return 0;
+}
+
+int apply_gate_list(hipmps_state_s* s, int n, const hipmps_gate* gates) {
+  for (int k = 0; k < n; ++k) {
+    const hipmps_gate& g = gates[k];
+    cd M[16];
+    if (g.nq == 1) {
+      if (g.q0 < 0 || g.q0 >= s->n) return fail(HIPMPS_E_ARG, "qubit index out of range");
+      for (int t = 0; t < 4; ++t) M[t] = cd(g.m[2 * t], g.m[2 * t + 1]);
+      queue_1q(s, g.q0, M);
+    } else if (g.nq == 2) {
+      if (g.q0 < 0 || g.q0 >= s->n || g.q1 < 0 || g.q1 >= s->n || g.q0 == g.q1)
+        return fail(HIPMPS_E_ARG, "qubit index out of range");
+      if (std::abs(g.q0 - g.q1) != 1)
+        return fail(HIPMPS_E_NONADJACENT, "2-qubit gate on non-neighbouring qubits " + std::to_string(g.q0) +
+                                              "," + std::to_string(g.q1));
+      for (int t = 0; t < 16; ++t) M[t] = cd(g.m[2 * t], g.m[2 * t + 1]);
+      bool conflict = false;
+      RC_TRY(queue_2q(s, g.q0, g.q1, M, &conflict));
+      if (conflict) {
+        RC_TRY(state_run_wave(s));
+        RC_TRY(queue_2q(s, g.q0, g.q1, M, &conflict));
+      }
+    } else {
+      return fail(HIPMPS_E_ARG, "gate must act on 1 or 2 qubits");
+    }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// measurement: environments
+// ------------------------------------------------------------------------------------
+struct EnvJob {
+  const void* A;
+  int cl, cr;
+  const double2* Ein;
+  double2* X;
+  double2* Eout;
+  int z;
+};
+
+// left step: E'(cr x cr) = A^H D (E A), D = diag(+-1) on the physical index when z
+int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
+  if (jobs.empty()) return 0;
+  const int N = (int)jobs.size();
+  RC_TRY(E->hdesc.ensure(2 * N * sizeof(GemmTask)));
+  RC_TRY(E->desc.ensure(2 * N * sizeof(GemmTask)));
+  GemmTask* h = (GemmTask*)E->hdesc.p;
+  int t1 = 0, t2 = 0;
+  for (int q = 0; q < N; ++q) {
+    auto& j = jobs[q];
+    GemmTask& a = h[q];
+    memset(&a, 0, sizeof(a));
+    a.A = j.Ein; a.lda = j.cl;
+    a.B = j.A; a.ldb = 2 * j.cr;
+    a.C = j.X; a.ldc = 2 * j.cr;
+    a.M = j.cl; a.N = 2 * j.cr; a.K = j.cl;
+    a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
+    a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
+    GemmTask& b = h[N + q];
+    memset(&b, 0, sizeof(b));
+    b.A = j.A; b.lda = j.cr;
+    b.B = j.X; b.ldb = j.cr;
+    b.C = j.Eout; b.ldc = j.cr;
+    b.M = j.cr; b.N = j.cr; b.K = 2 * j.cl;
+    b.sign_div = j.z ? 1 : 0;
+    b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
+    b.tile_begin = t2; t2 += b.tiles_m * b.tiles_n;
+  }
+  GemmTask* d = (GemmTask*)E->desc.p;
+  HIP_TRY(hipMemcpyAsync(d, h, 2 * N * sizeof(GemmTask), hipMemcpyHostToDevice, E->stream));
+  HIP_TRY(launch_gemm(d, N, t1, 0, 0, 0, fp32, 0, E->stream));
+  HIP_TRY(launch_gemm(d + N, N, t2, 2, 0, fp32, 0, 0, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));  // staging reuse
+  return 0;
+}
+
+// right step: E'(cl x cl) = conj(A) (A R^T)^T with the physical sign when z
+int env_right(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
+  if (jobs.empty()) return 0;
+  const int N = (int)jobs.size();
+  RC_TRY(E->hdesc.ensure(2 * N * sizeof(GemmTask)));
+  RC_TRY(E->desc.ensure(2 * N * sizeof(GemmTask)));
+  GemmTask* h = (GemmTask*)E->hdesc.p;
+  int t1 = 0, t2 = 0;
+  for (int q = 0; q < N; ++q) {
+    auto& j = jobs[q];
+    GemmTask& a = h[q];
+    memset(&a, 0, sizeof(a));
+    a.A = j.A; a.lda = j.cr;                 // (2cl) x cr
+    a.B = j.Ein; a.ldb = j.cr;               // R^T
+    a.C = j.X; a.ldc = j.cr;
+    a.M = 2 * j.cl; a.N = j.cr; a.K = j.cr;
+    a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
+    a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
+    GemmTask& b = h[N + q];
+    memset(&b, 0, sizeof(b));
+    b.A = j.A; b.lda = 2 * j.cr;             // conj, cl x 2cr
+    b.B = j.X; b.ldb = 2 * j.cr;             // X viewed cl x 2cr, transposed
+    b.C = j.Eout; b.ldc = j.cl;
+    b.M = j.cl; b.N = j.cl; b.K = 2 * j.cr;
+    b.sign_div = j.z ? j.cr : 0;
+    b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
+    b.tile_begin = t2; t2 += b.tiles_m * b.tiles_n;
+  }
+  GemmTask* d = (GemmTask*)E->desc.p;
+  HIP_TRY(hipMemcpyAsync(d, h, 2 * N * sizeof(GemmTask), hipMemcpyHostToDevice, E->stream));
+  HIP_TRY(launch_gemm(d, N, t1, 0, 3, fp32, 0, 0, E->stream));
+  HIP_TRY(launch_gemm(d + N, N, t2, 1, 3, fp32, 0, 0, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  return 0;
+}
+
+// environment storage: all left (L_0..L_n) and right (R_0..R_n) environments + scratch
+struct EnvLayout {
+  std::vector<size_t> offL, offR;
+  size_t offX = 0, offT = 0, total = 0;
+};
+
+EnvLayout env_layout(const hipmps_state_s* s, size_t extra_mats, size_t extra_elems) {
+  EnvLayout L;
+  size_t off = 0;
+  L.offL.resize(s->n + 1);
+  L.offR.resize(s->n + 1);
+  int cmax = 1;
+  for (int k = 0; k <= s->n; ++k) cmax = std::max(cmax, s->chi[k]);
+  for (int k = 0; k <= s->n; ++k) {
+    L.offL[k] = off;
+    off = align_up(off + (size_t)s->chi[k] * s->chi[k] * 16);
+  }
+  for (int k = 0; k <= s->n; ++k) {
+    L.offR[k] = off;
+    off = align_up(off + (size_t)s->chi[k] * s->chi[k] * 16);
+  }
+  L.offX = off;
+  off = align_up(off + align_up((size_t)cmax * 2 * cmax * 16) * std::max<size_t>(1, extra_mats));
+  L.offT = off;
+  off = align_up(off + std::max<size_t>(extra_elems, 1) * 16);
+  L.total = off;
+  return L;
+}
+
+int compute_left_envs(hipmps_state_s* s, const EnvLayout& L, const std::vector<int>& zmask) {
+  hipmps_engine E = s->e;
+  char* base = (char*)E->env.p;
+  HIP_TRY(launch_fill_identity1((double2*)(base + L.offL[0]), E->stream));
+  for (int k = 0; k < s->n; ++k) {
+    std::vector<EnvJob> jobs(1);
+    jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offL[k]),
+               (double2*)(base + L.offX), (double2*)(base + L.offL[k + 1]), zmask.empty() ? 0 : zmask[k]};
+    RC_TRY(env_left(E, jobs, s->fp32));
+  }
+  return 0;
+}
+
+int compute_right_envs(hipmps_state_s* s, const EnvLayout& L) {
+  hipmps_engine E = s->e;
+  char* base = (char*)E->env.p;
+  HIP_TRY(launch_fill_identity1((double2*)(base + L.offR[s->n]), E->stream));
+  for (int k = s->n - 1; k >= 0; --k) {
+    std::vector<EnvJob> jobs(1);
+    jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offR[k + 1]),
+               (double2*)(base + L.offX), (double2*)(base + L.offR[k]), 0};
+    RC_TRY(env_right(E, jobs, s->fp32));
+  }
+  return 0;
+}
+
+int read_scalar(hipmps_engine E, const void* dptr, double2* out) {
+  HIP_TRY(hipMemcpyAsync(out, dptr, sizeof(double2), hipMemcpyDeviceToHost, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  return 0;
+}
+
+// sum over pairs M_t o R_t (elementwise), results to host
+int trace_products(hipmps_engine E, const std::vector<const double2*>& A, const std::vector<const double2*>& B,
+                   const std::vector<int>& dims, char* scratch, std::vector<double2>& out) {
+  const int N = (int)A.size();
+  if (N == 0) return 0;
+  size_t bytes = align_up(N * sizeof(void*)) * 2 + align_up(N * sizeof(int)) + N * sizeof(double2);
+  RC_TRY(E->hdesc.ensure(bytes));
+  char* h = (char*)E->hdesc.p;
+  const double2** ha = (const double2**)h;
+  const double2** hb = (const double2**)(h + align_up(N * sizeof(void*)));
+  int* hdim = (int*)(h + 2 * align_up(N * sizeof(void*)));
+  for (int q = 0; q < N; ++q) {
+    ha[q] = A[q];
+    hb[q] = B[q];
+    hdim[q] = dims[q];
+  }
+  size_t head = 2 * align_up(N * sizeof(void*)) + align_up(N * sizeof(int));
+  HIP_TRY(hipMemcpyAsync(scratch, h, head, hipMemcpyHostToDevice, E->stream));
+  double2* dout = (double2*)(scratch + head);
+  HIP_TRY(launch_trace_prod((const double2* const*)scratch, (const double2* const*)(scratch + align_up(N * sizeof(void*))),
+                            (const int*)(scratch + 2 * align_up(N * sizeof(void*))), dout, N, E->stream));
+  out.resize(N);
+  HIP_TRY(hipMemcpyAsync(out.data(), dout, N * sizeof(double2), hipMemcpyDeviceToHost, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  return 0;
+}
+
+int measure_z_all(hipmps_state_s* s, double* out) {
+  hipmps_engine E = s->e;
+  const int n = s->n;
+  int cmax = 1;
+  for (int k = 0; k <= n; ++k) cmax = std::max(cmax, s->chi[k]);
+  // M_i buffers (n of them) + pointer tables
+  size_t mbytes = 0;
+  std::vector<size_t> offM(n);
+  for (int i = 0; i < n; ++i) {
+    offM[i] = mbytes;
+    mbytes = align_up(mbytes + (size_t)s->chi[i + 1] * s->chi[i + 1] * 16);
+  }
+  EnvLayout L = env_layout(s, (size_t)n, mbytes / 16 + 4096);
+  RC_TRY(E->env.ensure(L.total));
+  char* base = (char*)E->env.p;
+  RC_TRY(compute_left_envs(s, L, {}));
+  RC_TRY(compute_right_envs(s, L));
+  // M_i = env update of L_i with Z at i (batched); X scratch per job
+  std::vector<EnvJob> jobs;
+  size_t xstride = align_up((size_t)cmax * 2 * cmax * 16);
+  for (int i = 0; i < n; ++i)
+    jobs.push_back({s->sites[i].p, s->chi[i], s->chi[i + 1], (const double2*)(base + L.offL[i]),
+                    (double2*)(base + L.offX + i * xstride), (double2*)(base + L.offT + offM[i]), 1});
+  RC_TRY(env_left(E, jobs, s->fp32));
+  std::vector<const double2*> A, B;
+  std::vector<int> dims;
+  for (int i = 0; i < n; ++i) {
+    A.push_back((const double2*)(base + L.offT + offM[i]));
+    B.push_back((const double2*)(base + L.offR[i + 1]));
+    dims.push_back(s->chi[i + 1]);
+  }
+  std::vector<double2> res;
+  // pointer tables go right after the M buffers
+  RC_TRY(trace_products(E, A, B, dims, base + L.offT + align_up(mbytes), res));
+  for (int i = 0; i < n; ++i) out[i] = res[i].x;
+  return 0;
+}
+
+int measure_zz(hipmps_state_s* s, const int* pairs, int k, double* out) {
+  hipmps_engine E = s->e;
+  const int n = s->n;
+  int cmax = 1;
+  for (int b = 0; b <= n; ++b) cmax = std::max(cmax, s->chi[b]);
+  size_t mat = align_up((size_t)cmax * cmax * 16);
+  size_t xmat = align_up((size_t)cmax * 2 * cmax * 16);
+  EnvLayout L = env_layout(s, 1, 0);
+  size_t per_pair = 2 * mat + xmat;
+  size_t total = L.total + per_pair * (size_t)k + 4096 + (size_t)k * 64;
+  RC_TRY(E->env.ensure(total));
+  char* base = (char*)E->env.p;
+  RC_TRY(compute_left_envs(s, L, {}));
+  RC_TRY(compute_right_envs(s, L));
+  char* pb = base + L.total;
+  struct P { int i, j, cur; int pp; };
+  std::vector<P> ps(k);
+  std::vector<double> norm_val(1, 0.0);
+  double2 nrm;
+  RC_TRY(read_scalar(E, base + L.offL[n], &nrm));
+  for (int t = 0; t < k; ++t) {
+    int a = pairs[2 * t], b = pairs[2 * t + 1];
+    if (a < 0 || a >= n || b < 0 || b >= n) return fail(HIPMPS_E_ARG, "zz pair out of range");
+    if (a > b) std::swap(a, b);
+    ps[t] = {a, b, a, 0};
+  }
+  // advance all pairs in lock-step
+  bool any = true;
+  while (any) {
+    any = false;
+    std::vector<EnvJob> jobs;
+    std::vector<int> who;
+    for (int t = 0; t < k; ++t) {
+      P& p = ps[t];
+      if (p.i == p.j || p.cur > p.j) continue;
+      any = true;
+      char* buf = pb + per_pair * t;
+      const double2* ein = (p.cur == p.i) ? (const double2*)(base + L.offL[p.i]) : (const double2*)(buf + mat * p.pp);
+      double2* eout = (double2*)(buf + mat * (1 - p.pp));
+      int z = (p.cur == p.i || p.cur == p.j) ? 1 : 0;
+      jobs.push_back({s->sites[p.cur].p, s->chi[p.cur], s->chi[p.cur + 1], ein, (double2*)(buf + 2 * mat), eout, z});
+      who.push_back(t);
+    }
+    if (!any) break;
+    RC_TRY(env_left(E, jobs, s->fp32));
+    for (int t : who) {
+      ps[t].cur++;
+      ps[t].pp = 1 - ps[t].pp;
+    }
+  }
+  std::vector<const double2*> A, B;
+  std::vector<int> dims, idx;
+  for (int t = 0; t < k; ++t) {
+    P& p = ps[t];
+    if (p.i == p.j) {
+      out[t] = nrm.x;  // Z_i Z_i = I
+      continue;
+    }
+    A.push_back((const double2*)(pb + per_pair * t + mat * p.pp));
+    B.push_back((const double2*)(base + L.offR[p.j + 1]));
+    dims.push_back(s->chi[p.j + 1]);
+    idx.push_back(t);
+  }
+  std::vector<double2> res;
+  RC_TRY(trace_products(E, A, B, dims, pb + per_pair * k, res));
+  for (size_t q = 0; q < idx.size(); ++q) out[idx[q]] = res[q].x;
+  return 0;
+}
+
+int measure_parity(hipmps_state_s* s, const std::vector<int>& zmask, double* out) {
+  hipmps_engine E = s->e;
+  EnvLayout L = env_layout(s, 1, 0);
+  RC_TRY(E->env.ensure(L.total));
+  RC_TRY(compute_left_envs(s, L, zmask));
+  double2 v;
+  RC_TRY(read_scalar(E, (char*)E->env.p + L.offL[s->n], &v));
+  *out = v.x;
+  return 0;
+}
+
+// amplitude / slice chain
+int chain_contract(hipmps_state_s* s, int rows0, const int8_t* d_bits, const int* slice_bits, std::vector<double2>& out) {
+  hipmps_engine E = s->e;
+  const int n = s->n;
+  // rows after each site
+  std::vector<long long> rows(n + 1);
+  rows[0] = rows0;
+  for (int k = 0; k < n; ++k) rows[k + 1] = rows[k] * ((slice_bits && slice_bits[k] < 0) ? 2 : 1);
+  size_t maxv = 1;
+  for (int k = 0; k <= n; ++k) maxv = std::max(maxv, (size_t)rows[k] * s->chi[k]);
+  if (maxv > ((size_t)1 << 31)) return fail(HIPMPS_E_UNSUPPORTED, "slice too large");
+  size_t vb = align_up(maxv * 16);
+  RC_TRY(E->env.ensure(2 * vb + 4096));
+  char* base = (char*)E->env.p;
+  double2* v0 = (double2*)base;
+  double2* v1 = (double2*)(base + vb);
+  // v0 = ones (rows0 x 1)
+  {
+    std::vector<double2> ones(rows0, make_double2(1.0, 0.0));
+    HIP_TRY(hipMemcpyAsync(v0, ones.data(), rows0 * sizeof(double2), hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+  }
+  RC_TRY(E->hdesc.ensure(sizeof(ChainTask)));
+  RC_TRY(E->desc.ensure(sizeof(ChainTask)));
+  for (int k = 0; k < n; ++k) {
+    ChainTask* h = (ChainTask*)E->hdesc.p;
+    memset(h, 0, sizeof(*h));
+    h->vin = v0;
+    h->vout = v1;
+    h->A = s->sites[k].p;
+    h->cl = s->chi[k];
+    h->cr = s->chi[k + 1];
+    h->rows_in = (int)rows[k];
+    h->rows_out = (int)rows[k + 1];
+    h->bits = d_bits;
+    h->n = n;
+    h->site = k;
+    h->mode = slice_bits ? 1 : 0;
+    h->slice_bit = slice_bits ? slice_bits[k] : 0;
+    h->block_begin = 0;
+    h->blocks = (int)rows[k + 1] * ((h->cr + 255) / 256);
+    HIP_TRY(hipMemcpyAsync(E->desc.p, h, sizeof(ChainTask), hipMemcpyHostToDevice, E->stream));
+    HIP_TRY(launch_chain((const ChainTask*)E->desc.p, 1, h->blocks, s->fp32, E->stream));
+    HIP_TRY(hipStreamSynchronize(E->stream));
+    std::swap(v0, v1);
+  }
+  out.resize(rows[n]);
+  HIP_TRY(hipMemcpyAsync(out.data(), v0, rows[n] * sizeof(double2), hipMemcpyDeviceToHost, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));
+  return 0;
+}
+
+}  // namespace
+
+// ======================================================================================
+// C ABI
+// ======================================================================================
+extern "C" {
+
+const char* hipmps_last_error(void) { return t_err.c_str(); }
+const char* hipmps_version(void) { return "hipmps 0.1 (gfx950)"; }
+
+int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
+  if (!out) return fail(HIPMPS_E_ARG, "null output");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return fail(HIPMPS_E_DEVICE, "no HIP device available");
+  }
+  auto* e = new hipmps_engine_s();
+  e->device = opts ? opts->device : 0;
+  if (e->device < 0 || e->device >= ndev) {
+    delete e;
+    return fail(HIPMPS_E_ARG, "device ordinal out of range");
+  }
+  if (hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(HIPMPS_E_DEVICE, "stream creation failed");
+  }
+  size_t freeb = 0, totalb = 0;
+  (void)hipMemGetInfo(&freeb, &totalb);
+  e->budget = (opts && opts->hbm_budget_bytes) ? opts->hbm_budget_bytes : (size_t)(0.9 * (double)totalb);
+  if (opts && opts->jacobi_max_sweeps > 0) e->max_sweeps = opts->jacobi_max_sweeps;
+  if (opts && opts->jacobi_tol > 0) e->tol = opts->jacobi_tol;
+  for (auto& ev : e->ev) (void)hipEventCreate(&ev);
+  *out = e;
+  return 0;
+}
+
+int hipmps_engine_destroy(hipmps_engine e) {
+  if (!e) return 0;
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->stream);
+  // every submitted circuit that was not polled yet is owned through by_ticket
+  for (auto& kv : e->by_ticket) {
+    if (kv.second->st) hipmps_state_destroy(kv.second->st);
+    delete kv.second;
+  }
+  e->by_ticket.clear();
+  e->running.clear();
+  e->queue.clear();
+  e->ws.release();
+  e->desc.release();
+  e->small.release();
+  e->env.release();
+  e->hdesc.release();
+  e->hsmall.release();
+  for (auto& ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(e->stream);
+  delete e;
+  return 0;
+}
+
+int hipmps_engine_set_timing(hipmps_engine e, int enabled) {
+  if (!e) return fail(HIPMPS_E_ARG, "null engine");
+  e->timing = enabled != 0;
+  return 0;
+}
+
+int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset) {
+  if (!e) return fail(HIPMPS_E_ARG, "null engine");
+  if (out) *out = e->prof;
+  if (reset) e->prof = hipmps_profile_t{};
+  return 0;
+}
+
+int hipmps_state_create(hipmps_engine e, int n_qubits, const hipmps_trunc_opts* t, hipmps_state* out) {
+  if (!e || !out || n_qubits < 1) return fail(HIPMPS_E_ARG, "bad arguments to hipmps_state_create");
+  *out = nullptr;
+  RC_TRY(set_device(e));
+  auto* s = new hipmps_state_s();
+  s->e = e;
+  s->n = n_qubits;
+  if (t) s->t = *t;
+  else {
+    s->t.svd_cutoff = DBL_MIN;
+    s->t.max_bond_dim = 0;
+  }
+  s->fp32 = s->t.precision == HIPMPS_MIXED;
+  s->es = s->fp32 ? 8 : 16;
+  s->sites.resize(n_qubits);
+  int rc = state_init_product(s);
+  if (rc) {
+    hipmps_state_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return 0;
+}
+
+int hipmps_state_destroy(hipmps_state s) {
+  if (!s) return 0;
+  (void)hipSetDevice(s->e->device);
+  (void)hipStreamSynchronize(s->e->stream);
+  release_deferred(s);
+  for (auto& S : s->sites) {
+    if (S.p) (void)hipFree(S.p);
+    s->e->used -= S.cap * s->es;
+  }
+  delete s;
+  return 0;
+}
+
+int hipmps_state_reset(hipmps_state s) {
+  if (!s) return fail(HIPMPS_E_ARG, "null state");
+  RC_TRY(set_device(s->e));
+  return state_init_product(s);
+}
+
+int hipmps_apply_1q(hipmps_state s, int q, const double m[8]) {
+  if (!s || !m) return fail(HIPMPS_E_ARG, "null argument");
+  hipmps_gate g{};
+  g.nq = 1;
+  g.q0 = q;
+  memcpy(g.m, m, 8 * sizeof(double));
+  RC_TRY(set_device(s->e));
+  return apply_gate_list(s, 1, &g);
+}
+
+int hipmps_apply_2q(hipmps_state s, int q0, int q1, const double m[32]) {
+  if (!s || !m) return fail(HIPMPS_E_ARG, "null argument");
+  hipmps_gate g{};
+  g.nq = 2;
+  g.q0 = q0;
+  g.q1 = q1;
+  memcpy(g.m, m, 32 * sizeof(double));
+  RC_TRY(set_device(s->e));
+  return apply_gate_list(s, 1, &g);
+}
+
+int hipmps_apply_gates(hipmps_state s, int n, const hipmps_gate* gates) {
+  if (!s || (n > 0 && !gates)) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(set_device(s->e));
+  return apply_gate_list(s, n, gates);
+}
+
+int hipmps_flush(hipmps_state s) {
+  if (!s) return fail(HIPMPS_E_ARG, "null state");
+  RC_TRY(state_flush(s));
+  return s->status;
+}
+
+int hipmps_bond_dims(hipmps_state s, int* out) {
+  if (!s || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  for (int k = 0; k <= s->n; ++k) out[k] = s->chi[k];
+  return 0;
+}
+
+int hipmps_norm(hipmps_state s, double* out) {
+  if (!s || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  return measure_parity(s, {}, out);
+}
+
+int hipmps_stats(hipmps_state s, hipmps_stats_t* out) {
+  if (!s || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  memset(out, 0, sizeof(*out));
+  RC_TRY(measure_parity(s, {}, &out->norm));
+  out->discarded_weight = s->discarded;
+  out->fidelity_est = s->fid;
+  out->max_bond = s->maxb;
+  out->n_2q = s->n2q;
+  out->n_1q = s->n1q;
+  out->jacobi_sweeps_max = s->sweeps_max;
+  out->jacobi_sweeps_total = s->sweeps_total;
+  out->device_bytes = s->bytes;
+  return s->status;
+}
+
+int hipmps_expval_z_parity(hipmps_state s, const int* qubits, int k, double* out) {
+  if (!s || !out || (k > 0 && !qubits)) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  std::vector<int> zmask(s->n, 0);
+  for (int t = 0; t < k; ++t) {
+    if (qubits[t] < 0 || qubits[t] >= s->n) return fail(HIPMPS_E_ARG, "qubit out of range");
+    zmask[qubits[t]] ^= 1;  // parity over the listed bits (:618-629)
+  }
+  return measure_parity(s, zmask, out);
+}
+
+int hipmps_expval_z_all(hipmps_state s, double* out) {
+  if (!s || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  return measure_z_all(s, out);
+}
+
+int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out) {
+  if (!s || !out || (k > 0 && !pairs)) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  if (k == 0) return 0;
+  return measure_zz(s, pairs, k, out);
+}
+
+int hipmps_amplitudes(hipmps_state s, const int8_t* bits, int k, double* out) {
+  if (!s || !bits || !out || k <= 0) return fail(HIPMPS_E_ARG, "bad arguments");
+  RC_TRY(state_flush(s));
+  for (size_t t = 0; t < (size_t)k * s->n; ++t)
+    if (bits[t] != 0 && bits[t] != 1) return fail(HIPMPS_E_ARG, "bits must be 0 or 1");
+  hipmps_engine E = s->e;
+  // bits live after the chain buffers; upload into a dedicated small buffer
+  DevBuf db;
+  RC_TRY(db.ensure((size_t)k * s->n));
+  HIP_TRY(hipMemcpy(db.p, bits, (size_t)k * s->n, hipMemcpyHostToDevice));
+  std::vector<double2> v;
+  int rc = chain_contract(s, k, (const int8_t*)db.p, nullptr, v);
+  db.release();
+  (void)E;
+  if (rc) return rc;
+  for (int t = 0; t < k; ++t) {
+    out[2 * t] = v[t].x;
+    out[2 * t + 1] = v[t].y;
+  }
+  return 0;
+}
+
+int hipmps_wavefunction_slice(hipmps_state s, const int* bits, double* out, size_t cap) {
+  if (!s || !bits || !out) return fail(HIPMPS_E_ARG, "bad arguments");
+  RC_TRY(state_flush(s));
+  int open = 0;
+  for (int k = 0; k < s->n; ++k) {
+    if (bits[k] < -1 || bits[k] > 1) return fail(HIPMPS_E_ARG, "bits must be 0, 1 or -1");
+    open += bits[k] < 0;
+  }
+  if (open > 30) return fail(HIPMPS_E_UNSUPPORTED, "too many open legs");
+  if (cap < ((size_t)1 << open)) return fail(HIPMPS_E_ARG, "output buffer too small");
+  std::vector<double2> v;
+  RC_TRY(chain_contract(s, 1, nullptr, bits, v));
+  for (size_t t = 0; t < v.size(); ++t) {
+    out[2 * t] = v[t].x;
+    out[2 * t + 1] = v[t].y;
+  }
+  return 0;
+}
+
+int hipmps_state_vector(hipmps_state s, double* out, size_t cap) {
+  if (!s || !out) return fail(HIPMPS_E_ARG, "bad arguments");
+  if (s->n > 26) return fail(HIPMPS_E_UNSUPPORTED, "state vector limited to 26 qubits");
+  std::vector<int> bits(s->n, -1);
+  return hipmps_wavefunction_slice(s, bits.data(), out, cap);
+}
+
+int hipmps_get_site(hipmps_state s, int k, double* out, size_t cap) {
+  if (!s || !out || k < 0 || k >= s->n) return fail(HIPMPS_E_ARG, "bad arguments");
+  RC_TRY(state_flush(s));
+  size_t n = (size_t)s->chi[k] * 2 * s->chi[k + 1];
+  if (cap < n) return fail(HIPMPS_E_ARG, "output buffer too small");
+  hipmps_engine E = s->e;
+  if (s->fp32) {
+    std::vector<float> h(2 * n);
+    HIP_TRY(hipMemcpy(h.data(), s->sites[k].p, n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 2 * n; ++i) out[i] = h[i];
+  } else {
+    HIP_TRY(hipMemcpy(out, s->sites[k].p, n * 16, hipMemcpyDeviceToHost));
+  }
+  (void)E;
+  return 0;
+}
+
+int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* in) {
+  if (!s || !in || k < 0 || k >= s->n || chi_l < 1 || chi_r < 1) return fail(HIPMPS_E_ARG, "bad arguments");
+  if ((k == 0 && chi_l != 1) || (k == s->n - 1 && chi_r != 1)) return fail(HIPMPS_E_ARG, "boundary bonds must be 1");
+  RC_TRY(state_flush(s));
+  size_t n = (size_t)chi_l * 2 * chi_r;
+  RC_TRY(ensure_site(s, k, n, false, nullptr));
+  s->chi[k] = chi_l;
+  s->chi[k + 1] = chi_r;
+  s->maxb = std::max(s->maxb, std::max(chi_l, chi_r));
+  return upload_site_host(s, k, reinterpret_cast<const cd*>(in), n);
+}
+
+// ---------------------------------------------------------------------------------------
+// batches and serving
+// ---------------------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+int finish_circuit(PendingCircuit* pc) {
+  hipmps_state_s* s = pc->st;
+  hipmps_result* r = pc->res;
+  RC_TRY(state_flush(s));
+  if (r) {
+    if (r->expval_z) RC_TRY(measure_z_all(s, r->expval_z));
+    if (r->expval_zz && !pc->zz.empty()) RC_TRY(measure_zz(s, pc->zz.data(), (int)pc->zz.size() / 2, r->expval_zz));
+    RC_TRY(measure_parity(s, {}, &r->norm));
+    r->discarded_weight = s->discarded;
+    r->fidelity_est = s->fid;
+    r->max_bond = s->maxb;
+    r->status = s->status;
+    r->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pc->t0).count();
+  }
+  pc->done = true;
+  return 0;
+}
+
+// Feed each running circuit up to its next wave conflict, then run all waves together.
+int advance_all(hipmps_engine E, std::vector<PendingCircuit*>& run) {
+  std::vector<WaveItem> items;
+  for (auto* pc : run) {
+    if (pc->done) continue;
+    hipmps_state_s* s = pc->st;
+    while (pc->cursor < pc->gates.size()) {
+      const hipmps_gate& g = pc->gates[pc->cursor];
+      if (g.nq == 2) {
+        const int i = std::min(g.q0, g.q1);
+        if (s->busy[i] || s->busy[i + 1]) break;  // next wave
+      }
+      RC_TRY(apply_gate_list(s, 1, &g));
+      pc->cursor++;
+    }
+    for (auto& q : s->wave) {
+      WaveItem it{};
+      it.s = s;
+      it.i = q.i;
+      it.G = q.G;
+      items.push_back(it);
+    }
+  }
+  RC_TRY(run_wave(E, items));
+  for (auto* pc : run) {
+    if (pc->done) continue;
+    hipmps_state_s* s = pc->st;
+    s->wave.clear();
+    std::fill(s->busy.begin(), s->busy.end(), 0);
+    if (pc->cursor >= pc->gates.size()) RC_TRY(finish_circuit(pc));
+  }
+  return 0;
+}
+
+int start_circuit(hipmps_engine E, PendingCircuit* pc) {
+  hipmps_state st = nullptr;
+  RC_TRY(hipmps_state_create(E, pc->c.n_qubits, &pc->c.trunc, &st));
+  pc->st = st;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits, hipmps_result* results) {
+  if (!e || n < 0 || (n > 0 && (!circuits || !results))) return fail(HIPMPS_E_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(e->mu);
+  RC_TRY(set_device(e));
+  std::vector<PendingCircuit*> run;
+  int rc = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int k = 0; k < n && rc == 0; ++k) {
+    auto* pc = new PendingCircuit();
+    pc->c = circuits[k];
+    pc->gates.assign(circuits[k].gates, circuits[k].gates + circuits[k].n_gates);
+    if (circuits[k].n_zz > 0) pc->zz.assign(circuits[k].zz_pairs, circuits[k].zz_pairs + 2 * circuits[k].n_zz);
+    pc->res = &results[k];
+    pc->t0 = t0;
+    run.push_back(pc);
+    rc = start_circuit(e, pc);
+  }
+  while (rc == 0) {
+    bool all_done = true;
+    for (auto* pc : run) all_done &= pc->done;
+    if (all_done) break;
+    rc = advance_all(e, run);
+  }
+  for (auto* pc : run) {
+    if (pc->st) hipmps_state_destroy(pc->st);
+    delete pc;
+  }
+  return rc;
+}
+
+int hipmps_serve_submit(hipmps_engine e, const hipmps_circuit* c, hipmps_result* result, uint64_t* ticket) {
+  if (!e || !c || !result || !ticket) return fail(HIPMPS_E_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(e->mu);
+  auto* pc = new PendingCircuit();
+  pc->c = *c;
+  pc->gates.assign(c->gates, c->gates + c->n_gates);
+  if (c->n_zz > 0) pc->zz.assign(c->zz_pairs, c->zz_pairs + 2 * c->n_zz);
+  pc->res = result;
+  int maxb = c->trunc.max_bond_dim > 0 ? c->trunc.max_bond_dim : 4096;
+  pc->footprint = projected_bytes(c->n_qubits, maxb, c->trunc.precision == HIPMPS_MIXED) +
+                  (size_t)2 * (2 * maxb) * (2 * maxb) * 16;
+  if (pc->footprint > e->budget) {
+    delete pc;
+    return fail(HIPMPS_E_NOMEM, "circuit's projected footprint exceeds the HBM budget");
+  }
+  pc->t0 = std::chrono::steady_clock::now();
+  pc->ticket = e->next_ticket++;
+  *ticket = pc->ticket;
+  e->queue.push_back(pc);
+  e->by_ticket[pc->ticket] = pc;
+  return 0;
+}
+
+int hipmps_serve_step(hipmps_engine e, int* n_done) {
+  if (!e) return fail(HIPMPS_E_ARG, "null engine");
+  std::lock_guard<std::mutex> lock(e->mu);
+  RC_TRY(set_device(e));
+  // admission by projected HBM footprint
+  size_t inflight = 0;
+  for (auto* pc : e->running) inflight += pc->footprint;
+  while (!e->queue.empty()) {
+    PendingCircuit* pc = e->queue.front();
+    if (!e->running.empty() && inflight + pc->footprint > e->budget) break;
+    e->queue.pop_front();
+    RC_TRY(start_circuit(e, pc));
+    e->running.push_back(pc);
+    inflight += pc->footprint;
+  }
+  int done = 0;
+  if (!e->running.empty()) RC_TRY(advance_all(e, e->running));
+  std::vector<PendingCircuit*> keep;
+  for (auto* pc : e->running) {
+    if (pc->done) {
+      hipmps_state_destroy(pc->st);
+      pc->st = nullptr;
+      ++done;
+    } else {
+      keep.push_back(pc);
+    }
+  }
+  e->running.swap(keep);
+  if (n_done) *n_done = done;
+  return 0;
+}
+
+int hipmps_serve_poll(hipmps_engine e, uint64_t ticket, int* done) {
+  if (!e || !done) return fail(HIPMPS_E_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(e->mu);
+  auto it = e->by_ticket.find(ticket);
+  if (it == e->by_ticket.end()) return fail(HIPMPS_E_ARG, "unknown ticket");
+  *done = it->second->done ? 1 : 0;
+  if (it->second->done) {
+    delete it->second;
+    e->by_ticket.erase(it);
+  }
+  return 0;
+}
+
+int hipmps_serve_pending(hipmps_engine e, int* queued, int* running) {
+  if (!e) return fail(HIPMPS_E_ARG, "null engine");
+  std::lock_guard<std::mutex> lock(e->mu);
+  if (queued) *queued = (int)e->queue.size();
+  if (running) *running = (int)e->running.size();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// kernels.h -- device task descriptors and launchers of libhipmps (internal).
+//
+// All kernels work on batches described by task arrays that live in device memory,
+// so one launch serves every circuit / bond of a wave.  Site tensors are stored
+// A[l][p][r] row-major (r fastest) as complex128 (double2) or complex64 (float2).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hipmps {
+
+// Block size of the one-sided block-Jacobi SVD: a block pair is 2*JB rows.
+constexpr int JB = 16;
+constexpr int JP = 2 * JB;
+
+// theta' = G . (A_i x A_j): merge of two neighbouring sites and the 2-qubit gate
+// (ExaTnMpsVisitor.cpp:1401-1524), written as the (2cl) x (2cr) matrix with
+// rows (l, a) and columns (b, r).
+struct ThetaTask {
+  const void* Ai;    // cl x 2 x cm
+  const void* Aj;    // cm x 2 x cr
+  double2* theta;    // 2cl x 2cr, ld = 2cr
+  int cl, cm, cr;
+  int tile_begin;    // first block of this task in the grid
+  int tiles_l, tiles_r;
+  int pad[2];
+  double2 G[16];     // G[(a,b)*4 + (a',b')], a = bit of the left site
+};
+
+// One Jacobi problem on Y = [X | I_r] (r_pad x (c + r), row-major, ld = ldy): the rows
+// are orthogonalised with respect to their first c columns (the data X) and every
+// rotation is applied to all c + r columns, so the last r columns accumulate the
+// rotation product W^T (rows of the final Y: [sigma_k v_k^H | W[:,k]^T]).
+struct JacTask {
+  double2* Y;
+  int ldy, r, r_pad, c;
+  int c_tot;         // c + r
+  int nb;            // r_pad / JB (even)
+  int pair_begin;    // first block of this task in the grid
+  int* counter;      // rotated block pairs in the current sweep
+  int* visited;      // block pairs visited in the current sweep
+};
+
+// Everything needed after convergence: singular values, truncation, factors.
+struct SplitTask {
+  const double2* theta;  // m x n (ld n)
+  const double2* Y;      // r_pad x (c + r) (ld ldy): final rows [sigma_k x_k | w_k]
+  int m, n, r, c, ldy;
+  int transposed;        // 0: X = theta (m <= n); 1: X = theta^H (m > n)
+  double* rownorm;       // r
+  double* sig;           // r (sorted, descending)
+  int* perm;             // r
+  double* scale_y;       // K: factor built from the data part of row k
+  double* scale_w;       // K: factor built from the rotation part of row k
+  int* K;                // out: new bond dimension
+  double* wstats;        // out: [discarded, kept, total, renorm factor]
+  void* Ai;              // out site i: cl x 2 x K
+  void* Aj;              // out site j: K x 2 x cr
+  int cl, cr;
+  double cutoff;
+  int max_bond;
+  int pn_l2;
+  int absorb;            // 0 sqrt split, 1 all to the right factor
+  int renorm;
+  int site_fp32;         // site storage is complex64
+  int pad;
+};
+
+// Generic batched complex GEMM: C (M x N) = rowscale . op(A) op(B) . colscale
+// op codes: 0 N, 1 C (conjugate), 2 H (conjugate transpose), 3 T (transpose).
+struct GemmTask {
+  const void* A;
+  const void* B;
+  void* C;
+  int M, N, K;
+  int lda, ldb, ldc;
+  const int* Mdyn;        // if set, M = min(M, *Mdyn)
+  const int* Ndyn;        // if set, N = min(N, *Ndyn)
+  const int* gatherA;     // N/C: storage row of A for output row i
+  const int* gatherB;     // T/H: storage row of B for output column j
+  const double* rowscale; // per output row
+  const double* colscale; // per output column
+  int sign_div;           // >0: a(i,k) *= (-1)^((k / sign_div) & 1)
+  int tile_begin, tiles_m, tiles_n;
+  int accumulate;         // C += result
+};
+
+// Chain step for amplitudes / slices: rows of partial contractions v[row][l].
+struct ChainTask {
+  const double2* vin;   // rows_in x cl
+  double2* vout;        // rows_out x cr
+  const void* A;        // cl x 2 x cr
+  int cl, cr, rows_in, rows_out;
+  const int8_t* bits;   // amplitude mode: bits[row * n + site]; slice mode: unused
+  int n, site;
+  int slice_bit;        // slice mode: 0/1 fixed, -1 open (rows double)
+  int mode;             // 0 amplitude, 1 slice
+  int block_begin, blocks;
+};
+
+// ---- launchers (stream-ordered, return hipError_t) --------------------------------
+hipError_t launch_apply_1q(void* site, int fp32, int cl, int cr, const double2 m[4], hipStream_t s);
+hipError_t launch_theta(const ThetaTask* d_tasks, int ntasks, int total_tiles, int fp32,
+                        hipStream_t s);
+hipError_t launch_jacobi_prep(const SplitTask* d_split, const JacTask* d_jac, int ntasks,
+                              int max_rows, int max_cols, hipStream_t s);
+hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
+                               double tol, int inner_sweeps, hipStream_t s);
+hipError_t launch_rownorm(const SplitTask* d_split, int ntasks, int max_r, hipStream_t s);
+hipError_t launch_select(const SplitTask* d_split, int ntasks, hipStream_t s);
+hipError_t launch_gather(const SplitTask* d_split, int ntasks, int max_k, int max_len, int fp32,
+                         hipStream_t s);
+// element types: 0 = complex128, 1 = complex64
+hipError_t launch_gemm(const GemmTask* d_tasks, int ntasks, int total_tiles, int opa, int opb,
+                       int ta, int tb, int tc, hipStream_t s);
+hipError_t launch_chain(const ChainTask* d_tasks, int ntasks, int total_blocks, int fp32,
+                        hipStream_t s);
+hipError_t launch_trace_prod(const double2* const* d_a, const double2* const* d_b,
+                             const int* d_dims, double2* d_out, int ntasks, hipStream_t s);
+hipError_t launch_check_finite(const void* p, size_t n_complex, int fp32, int* d_flag,
+                               hipStream_t s);
+hipError_t launch_fill_identity1(double2* p, hipStream_t s);
+
+}  // namespace hipmps

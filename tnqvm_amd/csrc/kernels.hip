@@ -1,0 +1,839 @@
+// kernels.hip -- gfx950 kernels of libhipmps.
+//
+// Kernel map (reference function each one replaces, tnqvm/visitors/exatn-mps/):
+//   k_apply_1q ......... 1-qubit gate into a site      ExaTnMpsVisitor.cpp:1231-1273
+//   k_theta ............ merge + 2-qubit gate          ExaTnMpsVisitor.cpp:1401-1541
+//                        (+ stabilizeTensorBody, :141-161)
+//   k_jacobi_prep,
+//   k_jacobi_round ..... full SVD (one-sided block Jacobi, FP64)  :1555-1627
+//   k_rownorm, k_select  partial norms, cutoff, max-bond clamp    :2366-2536
+//   k_gather, k_gemm ... sqrt(S) split into the two new sites     :1623
+//   k_gemm, k_chain,
+//   k_trace_prod ....... environments / amplitudes / slices       :593-643, :2588-2675
+#include "kernels.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace hipmps {
+
+// ------------------------------------------------------------------ complex helpers
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// c + a*b
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+  c.x = fma(a.x, b.x, c.x);
+  c.x = fma(-a.y, b.y, c.x);
+  c.y = fma(a.x, b.y, c.y);
+  c.y = fma(a.y, b.x, c.y);
+  return c;
+}
+// c + conj(a)*b
+__device__ __forceinline__ double2 cfmac(double2 a, double2 b, double2 c) {
+  c.x = fma(a.x, b.x, c.x);
+  c.x = fma(a.y, b.y, c.x);
+  c.y = fma(a.x, b.y, c.y);
+  c.y = fma(-a.y, b.x, c.y);
+  return c;
+}
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+template <typename T>
+__device__ __forceinline__ double2 ld(const T* p, size_t i);
+template <>
+__device__ __forceinline__ double2 ld<double2>(const double2* p, size_t i) { return p[i]; }
+template <>
+__device__ __forceinline__ double2 ld<float2>(const float2* p, size_t i) {
+  float2 v = p[i];
+  return make_double2((double)v.x, (double)v.y);
+}
+template <typename T>
+__device__ __forceinline__ void st(T* p, size_t i, double2 v);
+template <>
+__device__ __forceinline__ void st<double2>(double2* p, size_t i, double2 v) { p[i] = v; }
+template <>
+__device__ __forceinline__ void st<float2>(float2* p, size_t i, double2 v) {
+  p[i] = make_float2((float)v.x, (float)v.y);
+}
+
+// Largest t with begin(t) <= b, for task arrays sorted by their first block.
+template <typename Task, typename F>
+__device__ __forceinline__ int find_task(const Task* tasks, int ntasks, int b, F begin) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (begin(tasks[mid]) <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------ 1-qubit gate
+// new[l][b][r] = sum_i M[b][i] A[l][i][r]  (pattern Result(a,b,c)=Q(a,i,c)*G(i,b),
+// ExaTnMpsVisitor.cpp:1246, with G(i,b) = M[b][i] from the row-major flattening)
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_1q(T* A, int cl, int cr, double2 m0, double2 m1,
+                                                  double2 m2, double2 m3) {
+  size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t total = (size_t)cl * cr;
+  if (idx >= total) return;
+  size_t l = idx / cr, r = idx % cr;
+  size_t i0 = (l * 2 + 0) * cr + r, i1 = (l * 2 + 1) * cr + r;
+  double2 a0 = ld(A, i0), a1 = ld(A, i1);
+  st(A, i0, cfma(m1, a1, cmul(m0, a0)));
+  st(A, i1, cfma(m3, a1, cmul(m2, a0)));
+}
+
+hipError_t launch_apply_1q(void* site, int fp32, int cl, int cr, const double2 m[4], hipStream_t s) {
+  size_t total = (size_t)cl * cr;
+  dim3 grid((unsigned)((total + 255) / 256));
+  if (fp32)
+    hipLaunchKernelGGL(k_apply_1q<float2>, grid, dim3(256), 0, s, (float2*)site, cl, cr, m[0], m[1], m[2], m[3]);
+  else
+    hipLaunchKernelGGL(k_apply_1q<double2>, grid, dim3(256), 0, s, (double2*)site, cl, cr, m[0], m[1], m[2], m[3]);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ theta = G (Ai Aj)
+// Tile: 16 values of l (32 rows (l,a)) x 32 values of r (64 columns (b,r)).
+// Thread: one l, two r (r, r+16), all four (a,b): 8 complex accumulators.
+constexpr int TH_L = 16, TH_R = 32, TH_K = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_theta(const ThetaTask* tasks, int ntasks) {
+  __shared__ double2 sAi[TH_L][2][TH_K + 1];
+  __shared__ double2 sAj[TH_K][2][TH_R + 1];
+  const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const ThetaTask& x) { return x.tile_begin; });
+  const ThetaTask& task = tasks[t];
+  const int tile = blockIdx.x - task.tile_begin;
+  if (tile >= task.tiles_l * task.tiles_r) return;
+  const int l0 = (tile / task.tiles_r) * TH_L, r0 = (tile % task.tiles_r) * TH_R;
+  const int cl = task.cl, cm = task.cm, cr = task.cr;
+  const T* Ai = (const T*)task.Ai;
+  const T* Aj = (const T*)task.Aj;
+  const int tid = threadIdx.x;
+  const int tl = tid / 16, tr = tid % 16;
+  double2 acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = make_double2(0, 0);
+
+  for (int m0 = 0; m0 < cm; m0 += TH_K) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int e = tid + 256 * q;  // 512 = 16 l x 2 a x 16 k
+      int l = e / 32, a = (e / 16) % 2, k = e % 16;
+      double2 v = make_double2(0, 0);
+      if (l0 + l < cl && m0 + k < cm) v = ld(Ai, ((size_t)(l0 + l) * 2 + a) * cm + m0 + k);
+      sAi[l][a][k] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int e = tid + 256 * q;  // 1024 = 16 k x 2 b x 32 r
+      int k = e / 64, b = (e / 32) % 2, rr = e % 32;
+      double2 v = make_double2(0, 0);
+      if (m0 + k < cm && r0 + rr < cr) v = ld(Aj, ((size_t)(m0 + k) * 2 + b) * cr + r0 + rr);
+      sAj[k][b][rr] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < TH_K; ++k) {
+      double2 a0 = sAi[tl][0][k], a1 = sAi[tl][1][k];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        double2 b0 = sAj[k][0][tr + 16 * x], b1 = sAj[k][1][tr + 16 * x];
+        acc[x][0] = cfma(a0, b0, acc[x][0]);
+        acc[x][1] = cfma(a0, b1, acc[x][1]);
+        acc[x][2] = cfma(a1, b0, acc[x][2]);
+        acc[x][3] = cfma(a1, b1, acc[x][3]);
+      }
+    }
+    __syncthreads();
+  }
+  const int l = l0 + tl;
+  if (l >= cl) return;
+  const size_t ldt = (size_t)2 * cr;
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    int r = r0 + tr + 16 * x;
+    if (r >= cr) continue;
+#pragma unroll
+    for (int ab = 0; ab < 4; ++ab) {
+      double2 o = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o = cfma(task.G[ab * 4 + k], acc[x][k], o);
+      // stabilizeTensorBody: |x| < 1e-100 -> 0 (ExaTnMpsVisitor.cpp:141-161)
+      if (cabs2(o) < 1e-200) o = make_double2(0, 0);
+      const int a = ab >> 1, b = ab & 1;
+      task.theta[((size_t)l * 2 + a) * ldt + (size_t)b * cr + r] = o;
+    }
+  }
+}
+
+hipError_t launch_theta(const ThetaTask* d_tasks, int ntasks, int total_tiles, int fp32, hipStream_t s) {
+  if (total_tiles <= 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(k_theta<float2>, dim3(total_tiles), dim3(256), 0, s, d_tasks, ntasks);
+  else
+    hipLaunchKernelGGL(k_theta<double2>, dim3(total_tiles), dim3(256), 0, s, d_tasks, ntasks);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Jacobi: prepare Y
+// Y = [X | I_r] with X = theta (m <= n) or theta^H (m > n); rows r..r_pad-1 are zero.
+__global__ __launch_bounds__(256) void k_jacobi_prep(const SplitTask* sp, const JacTask* jt) {
+  __shared__ double2 tile[32][33];
+  const SplitTask& S = sp[blockIdx.y];
+  const JacTask& J = jt[blockIdx.y];
+  const int tiles_c = (J.c_tot + 31) / 32;
+  const int tiles_r = J.r_pad / 32;
+  const int bt = blockIdx.x;
+  if (bt >= tiles_c * tiles_r) return;
+  const int i0 = (bt / tiles_c) * 32, c0 = (bt % tiles_c) * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  if (c0 >= J.c) {
+    // identity part (whole tile right of the data)
+    for (int y = ty; y < 32; y += 8) {
+      int i = i0 + y, c = c0 + tx;
+      if (c < J.c_tot) J.Y[(size_t)i * J.ldy + c] = make_double2((i < J.r && c - J.c == i) ? 1.0 : 0.0, 0.0);
+    }
+    return;
+  }
+  if (!S.transposed) {
+    for (int y = ty; y < 32; y += 8) {
+      int i = i0 + y, c = c0 + tx;
+      if (c < J.c_tot) {
+        double2 v = make_double2(0, 0);
+        if (c < J.c) {
+          if (i < J.r) v = S.theta[(size_t)i * S.n + c];
+        } else if (i < J.r && c - J.c == i) {
+          v = make_double2(1.0, 0.0);
+        }
+        J.Y[(size_t)i * J.ldy + c] = v;
+      }
+    }
+  } else {
+    // Y[i][c] = conj(theta[c][i]); read theta rows coalesced along i
+    for (int y = ty; y < 32; y += 8) {
+      int c = c0 + y, i = i0 + tx;
+      double2 v = make_double2(0, 0);
+      if (c < J.c && i < J.r) v = cconj(S.theta[(size_t)c * S.n + i]);
+      tile[y][tx] = v;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      int i = i0 + y, c = c0 + tx;
+      if (c < J.c_tot) {
+        double2 v = tile[tx][y];
+        if (c >= J.c) v = make_double2((i < J.r && c - J.c == i) ? 1.0 : 0.0, 0.0);
+        J.Y[(size_t)i * J.ldy + c] = v;
+      }
+    }
+  }
+}
+
+hipError_t launch_jacobi_prep(const SplitTask* d_split, const JacTask* d_jac, int ntasks, int max_rows,
+                              int max_cols, hipStream_t s) {  // max_cols: largest c + r
+  int tr = (max_rows + 31) / 32, tc = (max_cols + 31) / 32;
+  hipLaunchKernelGGL(k_jacobi_prep, dim3(tr * tc, ntasks), dim3(256), 0, s, d_split, d_jac);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ Jacobi: one round
+// One block per block pair (rows of two JB-row blocks).  Round-robin ("circle")
+// ordering: slot 0 holds block 0, slot s>0 holds block 1 + (s-1+round) mod (nb-1); pair
+// i couples slots i and nb-1-i, so every unordered block pair meets once per sweep.
+//
+// For the 2*JB rows:  Gram  Gam[i][j] = sum_c conj(Y[i][c]) Y[j][c]
+//                     Gam = W Lambda W^H  by cyclic two-sided Jacobi in LDS (FP64)
+//                     Y' = W^T Y          (rows become mutually orthogonal)
+// Rotation for a Hermitian pair (alpha, beta, gamma = |gamma| e^{i phi}), Golub & Van
+// Loan Alg. 8.4.1 with a phase: zeta = (beta-alpha)/(2|gamma|),
+// t = sign(zeta)/(|zeta|+sqrt(1+zeta^2)), c = 1/sqrt(1+t^2), s = c t,
+// J = [[c, s e^{i phi}], [-s e^{-i phi}, c]].
+__device__ __forceinline__ int circle_slot(int s, int round, int nb) {
+  return s == 0 ? 0 : 1 + (s - 1 + round) % (nb - 1);
+}
+
+__device__ __forceinline__ void jacobi_rotation(double alpha, double beta, double2 gamma, double tol,
+                                                double& c, double2& se) {
+  double g = sqrt(cabs2(gamma));
+  if (g == 0.0 || !(alpha > 0.0) || !(beta > 0.0) || g <= tol * sqrt(alpha * beta)) {
+    c = 1.0;
+    se = make_double2(0.0, 0.0);
+    return;
+  }
+  double zeta = (beta - alpha) / (2.0 * g);
+  double t;
+  if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+  else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  c = 1.0 / sqrt(1.0 + t * t);
+  double s = c * t;
+  se = make_double2(s * gamma.x / g, s * gamma.y / g);
+}
+
+__global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int ntasks, int round, double tol,
+                                                      int inner_sweeps) {
+  __shared__ double2 sY[JP][33];
+  __shared__ double2 sG[JP][JP + 1];
+  __shared__ double2 sW[JP][JP + 1];
+  __shared__ double s_c[JP / 2];
+  __shared__ double2 s_se[JP / 2];
+  __shared__ int s_p[JP / 2], s_q[JP / 2];
+  __shared__ double s_red[256];
+
+  const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const JacTask& x) { return x.pair_begin; });
+  const JacTask& T = tasks[t];
+  const int pi = blockIdx.x - T.pair_begin;
+  const int nb = T.nb;
+  if (pi >= nb / 2 || round >= nb - 1) return;
+  const int ba = circle_slot(pi, round, nb), bb = circle_slot(nb - 1 - pi, round, nb);
+  const int C = T.c;
+  const size_t ld = T.ldy;
+  double2* Y = T.Y;
+  const int tid = threadIdx.x;
+  auto grow = [&](int j) { return j < JB ? ba * JB + j : bb * JB + (j - JB); };
+
+  // ---- Gram ----
+  const int gi = tid / 8, gj0 = (tid % 8) * 4;
+  double2 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = make_double2(0, 0);
+  for (int c0 = 0; c0 < C; c0 += 32) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int e = tid + 256 * q;
+      int row = e / 32, col = e % 32;
+      double2 v = make_double2(0, 0);
+      if (c0 + col < C) v = Y[(size_t)grow(row) * ld + c0 + col];
+      sY[row][col] = v;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int cc = 0; cc < 32; ++cc) {
+      double2 yi = sY[gi][cc];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = cfmac(yi, sY[gj0 + q][cc], acc[q]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sG[gi][gj0 + q] = acc[q];
+  __syncthreads();
+
+  // ---- convergence test of this block pair ----
+  auto max_offdiag = [&]() -> double {
+    double m = 0.0;
+    for (int e = tid; e < JP * JP; e += 256) {
+      int i = e / JP, j = e % JP;
+      if (i == j) continue;
+      double aii = sG[i][i].x, ajj = sG[j][j].x;
+      if (aii > 0.0 && ajj > 0.0) m = fmax(m, sqrt(cabs2(sG[i][j]) / (aii * ajj)));
+    }
+    s_red[tid] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
+      __syncthreads();
+    }
+    double r = s_red[0];
+    __syncthreads();
+    return r;
+  };
+  if (tid == 0) atomicAdd(T.visited, 1);
+  if (max_offdiag() <= tol) return;
+
+  // ---- eigen-decomposition of the Gram block (cyclic Jacobi, FP64) ----
+  for (int e = tid; e < JP * JP; e += 256) sW[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int sw = 0; sw < inner_sweeps; ++sw) {
+    for (int step = 0; step < JP - 1; ++step) {
+      if (tid < JP / 2) {
+        int p = circle_slot(tid, step, JP), q = circle_slot(JP - 1 - tid, step, JP);
+        double c;
+        double2 se;
+        jacobi_rotation(sG[p][p].x, sG[q][q].x, sG[p][q], tol * 0.25, c, se);
+        s_p[tid] = p;
+        s_q[tid] = q;
+        s_c[tid] = c;
+        s_se[tid] = se;
+      }
+      __syncthreads();
+      // columns of G and W: x_p' = c x_p - conj(se) x_q ; x_q' = se x_p + c x_q
+      for (int u = tid; u < (JP / 2) * JP; u += 256) {
+        int k = u / JP, i = u % JP;
+        double c = s_c[k];
+        double2 se = s_se[k];
+        if (se.x == 0.0 && se.y == 0.0) continue;
+        int p = s_p[k], q = s_q[k];
+        double2 x = sG[i][p], y = sG[i][q];
+        sG[i][p] = cadd(cscale(x, c), cscale(cmul(cconj(se), y), -1.0));
+        sG[i][q] = cadd(cmul(se, x), cscale(y, c));
+        x = sW[i][p];
+        y = sW[i][q];
+        sW[i][p] = cadd(cscale(x, c), cscale(cmul(cconj(se), y), -1.0));
+        sW[i][q] = cadd(cmul(se, x), cscale(y, c));
+      }
+      __syncthreads();
+      // rows of G: G[p] = c G[p] - se G[q] ; G[q] = conj(se) G[p] + c G[q]
+      for (int u = tid; u < (JP / 2) * JP; u += 256) {
+        int k = u / JP, j = u % JP;
+        double c = s_c[k];
+        double2 se = s_se[k];
+        if (se.x == 0.0 && se.y == 0.0) continue;
+        int p = s_p[k], q = s_q[k];
+        double2 x = sG[p][j], y = sG[q][j];
+        sG[p][j] = cadd(cscale(x, c), cscale(cmul(se, y), -1.0));
+        sG[q][j] = cadd(cmul(cconj(se), x), cscale(y, c));
+      }
+      __syncthreads();
+    }
+    if (max_offdiag() <= tol * 0.25) break;
+  }
+
+  // ---- apply: Y'[i][c] = sum_j W[j][i] Y[j][c], over data and rotation columns ----
+  const int CT = T.c_tot;
+  const int ac = tid % 32, ai0 = (tid / 32) * 4;
+  for (int c0 = 0; c0 < CT; c0 += 32) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int e = tid + 256 * q;
+      int row = e / 32, col = e % 32;
+      double2 v = make_double2(0, 0);
+      if (c0 + col < CT) v = Y[(size_t)grow(row) * ld + c0 + col];
+      sY[row][col] = v;
+    }
+    __syncthreads();
+    double2 out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = make_double2(0, 0);
+    for (int j = 0; j < JP; ++j) {
+      double2 y = sY[j][ac];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[q] = cfma(sW[j][ai0 + q], y, out[q]);
+    }
+    if (c0 + ac < CT) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Y[(size_t)grow(ai0 + q) * ld + c0 + ac] = out[q];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) atomicAdd(T.counter, 1);
+}
+
+hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round, double tol,
+                               int inner_sweeps, hipStream_t s) {
+  if (total_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_jacobi_round, dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round, tol,
+                     inner_sweeps);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ row norms
+__global__ __launch_bounds__(256) void k_rownorm(const SplitTask* sp) {
+  const SplitTask& S = sp[blockIdx.y];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int i = blockIdx.x * 4 + wave;
+  if (i >= S.r) return;
+  double a = 0.0;
+  const double2* row = S.Y + (size_t)i * S.ldy;
+  for (int c = lane; c < S.c; c += 64) a += cabs2(row[c]);
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+  if (lane == 0) S.rownorm[i] = sqrt(a);
+}
+
+hipError_t launch_rownorm(const SplitTask* d_split, int ntasks, int max_r, hipStream_t s) {
+  hipLaunchKernelGGL(k_rownorm, dim3((max_r + 3) / 4, ntasks), dim3(256), 0, s, d_split);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ sort + truncation rule
+// Singular values sorted descending (bitonic in LDS, ties by row index).  Truncation
+// follows truncateSvdTensors (ExaTnMpsVisitor.cpp:2424-2445): with the sqrt(S) split
+// both partial norms of bond index k equal sigma_k (squared definition) or
+// sqrt(sigma_k) (2-norm definition); K = i+1 for the first i whose norm is below the
+// cutoff (the sub-threshold index itself is kept), else K0; then K = min(K, max_bond).
+constexpr int SEL_MAX = 4096;
+__global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
+  __shared__ double key[SEL_MAX];
+  __shared__ int idx[SEL_MAX];
+  __shared__ double red[256];
+  __shared__ int s_first;
+  const SplitTask& S = sp[blockIdx.x];
+  const int r = S.r;
+  int P = 1;
+  while (P < r) P <<= 1;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < P; i += 256) {
+    key[i] = i < r ? S.rownorm[i] : -1.0;
+    idx[i] = i;
+  }
+  if (tid == 0) s_first = r;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < P; i += 256) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          bool desc = (i & k) == 0;
+          double a = key[i], b = key[ixj];
+          int ia = idx[i], ib = idx[ixj];
+          // "a before b" in descending order with index tie-break
+          bool a_first = (a > b) || (a == b && ia < ib);
+          bool swap = desc ? !a_first : a_first;
+          if (swap) {
+            key[i] = b; key[ixj] = a;
+            idx[i] = ib; idx[ixj] = ia;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // first sub-threshold index
+  for (int i = tid; i < r; i += 256) {
+    double pn = S.pn_l2 ? sqrt(key[i]) : key[i];
+    if (pn < S.cutoff) atomicMin(&s_first, i);
+  }
+  __syncthreads();
+  int K = s_first < r ? s_first + 1 : r;
+  if (K > S.max_bond) K = S.max_bond;
+  if (K < 1) K = 1;
+  // kept / discarded weights (FP64)
+  double kept = 0.0, disc = 0.0;
+  for (int i = tid; i < r; i += 256) {
+    double w = key[i] * key[i];
+    if (i < K) kept += w; else disc += w;
+  }
+  red[tid] = kept;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) { if (tid < w) red[tid] += red[tid + w]; __syncthreads(); }
+  kept = red[0];
+  __syncthreads();
+  red[tid] = disc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) { if (tid < w) red[tid] += red[tid + w]; __syncthreads(); }
+  disc = red[0];
+  double f = 1.0;
+  if (S.renorm && kept > 0.0) f = sqrt((kept + disc) / kept);
+  // scales: left factor U (s)^aL, right factor (s)^aR V^H with s = f * sigma.
+  // Row k of Y is [sigma_k x_k | w_k]: the data part gives the factor on the x side
+  // (times sigma^-1), the rotation part the other factor (a unit vector).
+  const double aL = S.absorb == 0 ? 0.5 : 0.0, aR = 1.0 - aL;
+  const double a_y = S.transposed ? aL : aR;
+  const double a_w = S.transposed ? aR : aL;
+  for (int k = tid; k < r; k += 256) {
+    S.sig[k] = key[k];
+    S.perm[k] = idx[k];
+    if (k < K) {
+      double sg = key[k];
+      double gy = 0.0, gw = 0.0;
+      if (sg > 1e-150) {
+        double sf = sg * f;
+        gy = pow(sf, a_y) / sg;
+        gw = pow(sf, a_w);
+      } else {
+        gw = a_w == 0.0 ? 1.0 : 0.0;
+      }
+      S.scale_y[k] = gy;
+      S.scale_w[k] = gw;
+    }
+  }
+  if (tid == 0) {
+    *S.K = K;
+    S.wstats[0] = disc;
+    S.wstats[1] = kept;
+    S.wstats[2] = kept + disc;
+    S.wstats[3] = f;
+  }
+}
+
+hipError_t launch_select(const SplitTask* d_split, int ntasks, hipStream_t s) {
+  hipLaunchKernelGGL(k_select, dim3(ntasks), dim3(256), 0, s, d_split);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ factors from Y rows
+// blockIdx.z = 0: row gather     dst[k][j] = Y[perm k][off + j] * scale_k      (K x len)
+// blockIdx.z = 1: conj-transpose dst[i][k] = conj(Y[perm k][off + i]) * scale_k (len x K)
+// not transposed (X = theta):  Aj <- rows of the data part (scale_y),  Ai <- conj W (scale_w)
+// transposed (X = theta^H):    Aj <- rows of the W part (scale_w),     Ai <- conj data (scale_y)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const SplitTask* sp) {
+  __shared__ double2 tile[32][33];
+  const SplitTask& S = sp[blockIdx.y];
+  const int K = *S.K;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  const int bt = blockIdx.x;
+  if (blockIdx.z == 0) {
+    const int len = S.n;                       // Aj is K x n
+    const int off = S.transposed ? S.c : 0;
+    const double* sc = S.transposed ? S.scale_w : S.scale_y;
+    const int tiles_c = (len + 31) / 32;
+    const int k0 = (bt / tiles_c) * 32, c0 = (bt % tiles_c) * 32;
+    if (k0 >= K) return;
+    T* out = (T*)S.Aj;
+    for (int y = ty; y < 32; y += 8) {
+      int k = k0 + y, c = c0 + tx;
+      if (k < K && c < len) {
+        double2 v = S.Y[(size_t)S.perm[k] * S.ldy + off + c];
+        st(out, (size_t)k * len + c, cscale(v, sc[k]));
+      }
+    }
+  } else {
+    const int len = S.m;                       // Ai is m x K
+    const int off = S.transposed ? 0 : S.c;
+    const double* sc = S.transposed ? S.scale_y : S.scale_w;
+    const int tiles_i = (len + 31) / 32;
+    const int k0 = (bt / tiles_i) * 32, i0 = (bt % tiles_i) * 32;
+    if (k0 >= K) return;
+    T* out = (T*)S.Ai;
+    for (int y = ty; y < 32; y += 8) {
+      int k = k0 + y, i = i0 + tx;
+      double2 v = make_double2(0, 0);
+      if (k < K && i < len) v = cscale(cconj(S.Y[(size_t)S.perm[k] * S.ldy + off + i]), sc[k]);
+      tile[y][tx] = v;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8) {
+      int i = i0 + y, k = k0 + tx;
+      if (i < len && k < K) st(out, (size_t)i * K + k, tile[tx][y]);
+    }
+  }
+}
+
+hipError_t launch_gather(const SplitTask* d_split, int ntasks, int max_k, int max_len, int fp32,
+                         hipStream_t s) {
+  int tiles = ((max_k + 31) / 32) * ((max_len + 31) / 32);
+  dim3 grid(tiles, ntasks, 2);
+  if (fp32)
+    hipLaunchKernelGGL(k_gather<float2>, grid, dim3(256), 0, s, d_split);
+  else
+    hipLaunchKernelGGL(k_gather<double2>, grid, dim3(256), 0, s, d_split);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ generic batched GEMM
+// C[i][j] = rowscale_i * colscale_j * sum_k a(i,k) b(k,j)
+//   a(i,k): N  A[gA(i)*lda + k]   C conj(A[gA(i)*lda + k])
+//           H  conj(A[k*lda + i]) T A[k*lda + i]
+//   b(k,j): N  B[k*ldb + j]       T B[gB(j)*ldb + k]   H conj(B[gB(j)*ldb + k])
+// optional sign (-1)^((k / sign_div) & 1) on a(i,k) (a diagonal Z inserted on the k index).
+constexpr int GM = 32, GN = 32, GK = 16;
+
+template <typename TA, typename TB, typename TC, int OPA, int OPB>
+__global__ __launch_bounds__(256) void k_gemm(const GemmTask* tasks, int ntasks) {
+  __shared__ double2 sA[GM][GK + 1];
+  __shared__ double2 sB[GK][GN + 1];
+  const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const GemmTask& x) { return x.tile_begin; });
+  const GemmTask& T = tasks[t];
+  const int tile = blockIdx.x - T.tile_begin;
+  if (tile >= T.tiles_m * T.tiles_n) return;
+  int M = T.M, N = T.N;
+  if (T.Mdyn) M = min(M, *T.Mdyn);
+  if (T.Ndyn) N = min(N, *T.Ndyn);
+  const int i0 = (tile / T.tiles_n) * GM, j0 = (tile % T.tiles_n) * GN;
+  if (i0 >= M || j0 >= N) return;
+  const int Kd = T.K;
+  const int ldc = T.ldc > 0 ? T.ldc : N;
+  const TA* A = (const TA*)T.A;
+  const TB* B = (const TB*)T.B;
+  const int tid = threadIdx.x;
+  const int ty = tid / 16, tx = tid % 16;
+  double2 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = make_double2(0, 0);
+
+  for (int k0 = 0; k0 < Kd; k0 += GK) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int e = tid + 256 * q;  // 512 = 32 x 16
+      int i, k;
+      if (OPA == 0 || OPA == 1) { i = e / GK; k = e % GK; }
+      else { k = e / GM; i = e % GM; }
+      double2 v = make_double2(0, 0);
+      int gi = i0 + i, gk = k0 + k;
+      if (gi < M && gk < Kd) {
+        if (OPA == 0 || OPA == 1) {
+          int sr = T.gatherA ? T.gatherA[gi] : gi;
+          v = ld(A, (size_t)sr * T.lda + gk);
+        } else {
+          v = ld(A, (size_t)gk * T.lda + gi);
+        }
+        if (OPA == 1 || OPA == 2) v = cconj(v);
+        if (T.sign_div > 0 && ((gk / T.sign_div) & 1)) v = cscale(v, -1.0);
+      }
+      sA[i][k] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      int e = tid + 256 * q;  // 512 = 16 x 32
+      int k, j;
+      if (OPB == 0) { k = e / GN; j = e % GN; }
+      else { j = e / GK; k = e % GK; }
+      double2 v = make_double2(0, 0);
+      int gj = j0 + j, gk = k0 + k;
+      if (gj < N && gk < Kd) {
+        if (OPB == 0) {
+          v = ld(B, (size_t)gk * T.ldb + gj);
+        } else {
+          int sr = T.gatherB ? T.gatherB[gj] : gj;
+          v = ld(B, (size_t)sr * T.ldb + gk);
+          if (OPB == 2) v = cconj(v);
+        }
+      }
+      sB[k][j] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < GK; ++k) {
+      double2 a0 = sA[ty * 2][k], a1 = sA[ty * 2 + 1][k];
+      double2 b0 = sB[k][tx * 2], b1 = sB[k][tx * 2 + 1];
+      acc[0][0] = cfma(a0, b0, acc[0][0]);
+      acc[0][1] = cfma(a0, b1, acc[0][1]);
+      acc[1][0] = cfma(a1, b0, acc[1][0]);
+      acc[1][1] = cfma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  TC* Cp = (TC*)T.C;
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    int i = i0 + ty * 2 + x;
+    if (i >= M) continue;
+    double rs = T.rowscale ? T.rowscale[i] : 1.0;
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      int j = j0 + tx * 2 + y;
+      if (j >= N) continue;
+      double s = rs * (T.colscale ? T.colscale[j] : 1.0);
+      double2 v = cscale(acc[x][y], s);
+      size_t o = (size_t)i * ldc + j;
+      if (T.accumulate) v = cadd(v, ld(Cp, o));
+      st(Cp, o, v);
+    }
+  }
+}
+
+template <int OPA, int OPB>
+static hipError_t gemm_types(const GemmTask* d, int nt, int tiles, int ta, int tb, int tc, hipStream_t s) {
+  dim3 g(tiles), b(256);
+#define HIPMPS_GEMM_CASE(A_, B_, C_, a, b_, c)                                                   \
+  if (ta == a && tb == b_ && tc == c) {                                                          \
+    hipLaunchKernelGGL((k_gemm<A_, B_, C_, OPA, OPB>), g, b, 0, s, d, nt);                       \
+    return hipGetLastError();                                                                    \
+  }
+  HIPMPS_GEMM_CASE(double2, double2, double2, 0, 0, 0)
+  HIPMPS_GEMM_CASE(double2, double2, float2, 0, 0, 1)
+  HIPMPS_GEMM_CASE(float2, double2, double2, 1, 0, 0)
+  HIPMPS_GEMM_CASE(double2, float2, double2, 0, 1, 0)
+  HIPMPS_GEMM_CASE(float2, float2, double2, 1, 1, 0)
+#undef HIPMPS_GEMM_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_gemm(const GemmTask* d_tasks, int ntasks, int total_tiles, int opa, int opb, int ta, int tb,
+                       int tc, hipStream_t s) {
+  if (total_tiles <= 0) return hipSuccess;
+#define HIPMPS_OP_CASE(a, b)                                                                     \
+  if (opa == a && opb == b) return gemm_types<a, b>(d_tasks, ntasks, total_tiles, ta, tb, tc, s);
+  HIPMPS_OP_CASE(0, 0)
+  HIPMPS_OP_CASE(0, 2)
+  HIPMPS_OP_CASE(0, 3)
+  HIPMPS_OP_CASE(1, 3)
+  HIPMPS_OP_CASE(2, 0)
+#undef HIPMPS_OP_CASE
+  return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------ chain step (amplitudes / slices)
+// amplitude: vout[row][r] = sum_l vin[row][l] A[l][bits[row][site]][r]
+// slice:     vout[o][r]   = sum_l vin[o % rows_in][l] A[l][p][r], p = open ? o / rows_in : bit
+template <typename T>
+__global__ __launch_bounds__(256) void k_chain(const ChainTask* tasks, int ntasks) {
+  const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const ChainTask& x) { return x.block_begin; });
+  const ChainTask& C = tasks[t];
+  const int b = blockIdx.x - C.block_begin;
+  if (b >= C.blocks) return;
+  const int rblocks = (C.cr + 255) / 256;
+  const int row = b / rblocks, r = (b % rblocks) * 256 + threadIdx.x;
+  if (row >= C.rows_out || r >= C.cr) return;
+  int src, p;
+  if (C.mode == 0) {
+    src = row;
+    p = C.bits[(size_t)row * C.n + C.site];
+  } else {
+    src = row % C.rows_in;
+    p = C.slice_bit < 0 ? row / C.rows_in : C.slice_bit;
+  }
+  const T* A = (const T*)C.A;
+  const double2* v = C.vin + (size_t)src * C.cl;
+  double2 acc = make_double2(0, 0);
+  for (int l = 0; l < C.cl; ++l) acc = cfma(v[l], ld(A, ((size_t)l * 2 + p) * C.cr + r), acc);
+  C.vout[(size_t)row * C.cr + r] = acc;
+}
+
+hipError_t launch_chain(const ChainTask* d_tasks, int ntasks, int total_blocks, int fp32, hipStream_t s) {
+  if (total_blocks <= 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(k_chain<float2>, dim3(total_blocks), dim3(256), 0, s, d_tasks, ntasks);
+  else
+    hipLaunchKernelGGL(k_chain<double2>, dim3(total_blocks), dim3(256), 0, s, d_tasks, ntasks);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ sum_{ij} a_ij b_ij
+__global__ __launch_bounds__(256) void k_trace_prod(const double2* const* a, const double2* const* b,
+                                                    const int* dims, double2* out) {
+  __shared__ double2 red[256];
+  const int t = blockIdx.x;
+  const size_t n = (size_t)dims[t] * dims[t];
+  double2 acc = make_double2(0, 0);
+  for (size_t e = threadIdx.x; e < n; e += 256) acc = cfma(a[t][e], b[t][e], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[t] = red[0];
+}
+
+hipError_t launch_trace_prod(const double2* const* d_a, const double2* const* d_b, const int* d_dims,
+                             double2* d_out, int ntasks, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trace_prod, dim3(ntasks), dim3(256), 0, s, d_a, d_b, d_dims, d_out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ misc
+__global__ void k_check_finite(const double* p, size_t n, int* flag) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && !isfinite(p[i])) atomicOr(flag, 1);
+}
+__global__ void k_check_finite_f(const float* p, size_t n, int* flag) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n && !isfinite(p[i])) atomicOr(flag, 1);
+}
+hipError_t launch_check_finite(const void* p, size_t n_complex, int fp32, int* d_flag, hipStream_t s) {
+  size_t n = 2 * n_complex;
+  if (n == 0) return hipSuccess;
+  dim3 g((unsigned)((n + 255) / 256));
+  if (fp32) hipLaunchKernelGGL(k_check_finite_f, g, dim3(256), 0, s, (const float*)p, n, d_flag);
+  else hipLaunchKernelGGL(k_check_finite, g, dim3(256), 0, s, (const double*)p, n, d_flag);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_identity1(double2* p) { p[0] = make_double2(1.0, 0.0); }
+hipError_t launch_fill_identity1(double2* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_identity1, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError();
+}
+
+}  // namespace hipmps

@@ -1,0 +1,283 @@
+"""Python front end of libhipmps: engine, MPS state, batches and serving.
+
+This module is a thin layer over the C ABI (include/hipmps.h); every numeric
+operation runs in the HIP library.  Gate dicts follow tnqvm_amd.circuits.
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+import weakref
+
+import numpy as np
+
+from . import _capi as C
+from ._capi import check, lib
+
+DBL_MIN = sys.float_info.min
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def gate_matrix(name: str, params=()) -> np.ndarray:
+    """Row-major gate matrix from the library's gate table (2-qubit: first operand = MSB)."""
+    p = np.ascontiguousarray(np.asarray(params, dtype=np.float64).reshape(-1))
+    buf = np.zeros(32)
+    k = check(lib().hipmps_gate_matrix(name.encode(), p.ctypes.data_as(_dp) if p.size else None, int(p.size),
+                                       buf.ctypes.data_as(_dp)))
+    d = 2 if k == 1 else 4
+    return buf[: 2 * d * d].view(np.complex128).reshape(d, d).copy()
+
+
+def matrix_of(g) -> np.ndarray:
+    if "matrix" in g:
+        return np.ascontiguousarray(np.asarray(g["matrix"], dtype=np.complex128))
+    return gate_matrix(g["name"], g.get("params", ()))
+
+
+def to_c_gates(gates):
+    """Gate dicts -> ctypes array of hipmps_gate (Measure / I are skipped)."""
+    gl = [g for g in gates if g["name"] not in ("Measure", "I")]
+    arr = (C.Gate * max(1, len(gl)))()
+    for k, g in enumerate(gl):
+        m = matrix_of(g).reshape(-1)
+        q = g["qubits"]
+        arr[k].nq = len(q)
+        arr[k].q0 = int(q[0])
+        arr[k].q1 = int(q[1]) if len(q) > 1 else -1
+        flat = np.zeros(32)
+        flat[: 2 * m.size] = m.view(np.float64)
+        arr[k].m[:] = flat.tolist()
+    return arr, len(gl)
+
+
+def trunc_opts(cutoff=DBL_MIN, max_bond=0, precision="fp64", renormalize=False, partial_norm="squared",
+               absorb="sqrt") -> C.TruncOpts:
+    t = C.TruncOpts()
+    t.svd_cutoff = float(cutoff)
+    t.max_bond_dim = int(max_bond)
+    t.precision = C.HIPMPS_MIXED if precision == "mixed" else C.HIPMPS_FP64
+    t.renormalize = int(bool(renormalize))
+    t.partial_norm = C.HIPMPS_PN_L2 if partial_norm == "l2" else C.HIPMPS_PN_SQUARED
+    t.absorb = C.HIPMPS_ABSORB_RIGHT if absorb == "right" else C.HIPMPS_ABSORB_SQRT
+    return t
+
+
+class Engine:
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, jacobi_max_sweeps: int = 0,
+                 jacobi_tol: float = 0.0):
+        o = C.EngineOpts(device, hbm_budget_bytes, jacobi_max_sweeps, jacobi_tol)
+        h = ctypes.c_void_p()
+        check(lib().hipmps_engine_create(ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self._states = weakref.WeakSet()
+
+    def close(self):
+        if getattr(self, "h", None):
+            # states hold device memory owned through the engine: release them first
+            for s in list(self._states):
+                s.close()
+            lib().hipmps_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on: bool):
+        check(lib().hipmps_engine_set_timing(self.h, int(on)))
+
+    def profile(self, reset: bool = False) -> dict:
+        p = C.Profile()
+        check(lib().hipmps_engine_profile(self.h, ctypes.byref(p), int(reset)))
+        return {k: getattr(p, k) for k, _ in C.Profile._fields_}
+
+    # -- batches -------------------------------------------------------------------
+    def _circuit(self, n, gates, trunc, zz):
+        garr, ng = to_c_gates(gates)
+        c = C.Circuit()
+        c.n_qubits = n
+        c.n_gates = ng
+        c.gates = ctypes.cast(garr, ctypes.POINTER(C.Gate))
+        c.trunc = trunc
+        zarr = None
+        if zz:
+            zarr = np.ascontiguousarray(np.asarray(zz, dtype=np.int32).reshape(-1))
+            c.n_zz = len(zz)
+            c.zz_pairs = zarr.ctypes.data_as(_ip)
+        return c, (garr, zarr)
+
+    def run_batch(self, jobs):
+        """jobs: list of dicts {n, gates, trunc (TruncOpts), zz (list of pairs)}.
+        Returns list of dicts with expval_z, expval_zz, norm, discarded_weight, ..."""
+        keep = []
+        circs = (C.Circuit * len(jobs))()
+        res = (C.Result * len(jobs))()
+        outs = []
+        for k, j in enumerate(jobs):
+            c, hold = self._circuit(j["n"], j["gates"], j.get("trunc", trunc_opts()), j.get("zz"))
+            keep.append(hold)
+            circs[k] = c
+            ez = np.zeros(j["n"])
+            ezz = np.zeros(max(1, len(j.get("zz") or [])))
+            outs.append((ez, ezz))
+            res[k].expval_z = ez.ctypes.data_as(_dp)
+            res[k].expval_zz = ezz.ctypes.data_as(_dp)
+        check(lib().hipmps_run_batch(self.h, len(jobs), circs, res))
+        out = []
+        for k, j in enumerate(jobs):
+            r = res[k]
+            out.append({"expval_z": outs[k][0], "expval_zz": outs[k][1][: len(j.get("zz") or [])],
+                        "norm": r.norm, "discarded_weight": r.discarded_weight, "fidelity_est": r.fidelity_est,
+                        "max_bond": r.max_bond, "status": r.status, "latency_ms": r.latency_ms})
+        return out
+
+
+class Server:
+    """Continuous-batching request loop on top of hipmps_serve_*."""
+
+    def __init__(self, engine: Engine):
+        self.e = engine
+        self.live = {}
+
+    def submit(self, n, gates, trunc=None, zz=None) -> int:
+        c, hold = self.e._circuit(n, gates, trunc or trunc_opts(), zz)
+        res = C.Result()
+        ez = np.zeros(n)
+        ezz = np.zeros(max(1, len(zz or [])))
+        res.expval_z = ez.ctypes.data_as(_dp)
+        res.expval_zz = ezz.ctypes.data_as(_dp)
+        t = ctypes.c_uint64()
+        check(lib().hipmps_serve_submit(self.e.h, ctypes.byref(c), ctypes.byref(res), ctypes.byref(t)))
+        self.live[t.value] = (c, hold, res, ez, ezz, len(zz or []))
+        return t.value
+
+    def step(self) -> int:
+        d = ctypes.c_int()
+        check(lib().hipmps_serve_step(self.e.h, ctypes.byref(d)))
+        return d.value
+
+    def pending(self):
+        q, r = ctypes.c_int(), ctypes.c_int()
+        check(lib().hipmps_serve_pending(self.e.h, ctypes.byref(q), ctypes.byref(r)))
+        return q.value, r.value
+
+    def poll(self, ticket: int):
+        d = ctypes.c_int()
+        check(lib().hipmps_serve_poll(self.e.h, ticket, ctypes.byref(d)))
+        if not d.value:
+            return None
+        c, hold, res, ez, ezz, nzz = self.live.pop(ticket)
+        return {"expval_z": ez, "expval_zz": ezz[:nzz], "norm": res.norm, "discarded_weight": res.discarded_weight,
+                "fidelity_est": res.fidelity_est, "max_bond": res.max_bond, "status": res.status,
+                "latency_ms": res.latency_ms}
+
+
+class MpsState:
+    """One MPS circuit state resident in HBM (hipmps_state)."""
+
+    def __init__(self, engine: Engine, n: int, **trunc):
+        self.e = engine
+        self.n = n
+        self.trunc = trunc_opts(**trunc)
+        h = ctypes.c_void_p()
+        check(lib().hipmps_state_create(engine.h, n, ctypes.byref(self.trunc), ctypes.byref(h)))
+        self.h = h
+        engine._states.add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if getattr(self.e, "h", None):
+                lib().hipmps_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self):
+        check(lib().hipmps_state_reset(self.h))
+
+    def apply(self, g):
+        m = matrix_of(g)
+        q = g["qubits"]
+        flat = np.ascontiguousarray(m.reshape(-1)).view(np.float64)
+        if len(q) == 1:
+            check(lib().hipmps_apply_1q(self.h, int(q[0]), flat.ctypes.data_as(_dp)))
+        else:
+            check(lib().hipmps_apply_2q(self.h, int(q[0]), int(q[1]), flat.ctypes.data_as(_dp)))
+
+    def run(self, gates):
+        arr, ng = to_c_gates(gates)
+        check(lib().hipmps_apply_gates(self.h, ng, arr))
+        return self
+
+    def flush(self):
+        return check(lib().hipmps_flush(self.h))
+
+    def bonds(self) -> np.ndarray:
+        out = np.zeros(self.n + 1, dtype=np.int32)
+        check(lib().hipmps_bond_dims(self.h, out.ctypes.data_as(_ip)))
+        return out
+
+    def stats(self) -> dict:
+        s = C.Stats()
+        lib().hipmps_stats(self.h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in C.Stats._fields_}
+
+    def norm(self) -> float:
+        v = ctypes.c_double()
+        check(lib().hipmps_norm(self.h, ctypes.byref(v)))
+        return v.value
+
+    def expval_z_parity(self, qubits) -> float:
+        q = np.ascontiguousarray(np.asarray(qubits, dtype=np.int32))
+        v = ctypes.c_double()
+        check(lib().hipmps_expval_z_parity(self.h, q.ctypes.data_as(_ip), len(q), ctypes.byref(v)))
+        return v.value
+
+    def expval_z_all(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        check(lib().hipmps_expval_z_all(self.h, out.ctypes.data_as(_dp)))
+        return out
+
+    def expval_zz(self, pairs) -> np.ndarray:
+        p = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1))
+        out = np.zeros(len(p) // 2)
+        check(lib().hipmps_expval_zz(self.h, p.ctypes.data_as(_ip), len(p) // 2, out.ctypes.data_as(_dp)))
+        return out
+
+    def amplitudes(self, bits) -> np.ndarray:
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8))
+        out = np.zeros(b.shape[0], dtype=np.complex128)
+        check(lib().hipmps_amplitudes(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), b.shape[0],
+                                      out.view(np.float64).ctypes.data_as(_dp)))
+        return out
+
+    def slice(self, bits) -> np.ndarray:
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int32))
+        nopen = int((b < 0).sum())
+        out = np.zeros(1 << nopen, dtype=np.complex128)
+        check(lib().hipmps_wavefunction_slice(self.h, b.ctypes.data_as(_ip), out.view(np.float64).ctypes.data_as(_dp),
+                                              out.size))
+        return out
+
+    def state_vector(self) -> np.ndarray:
+        out = np.zeros(1 << self.n, dtype=np.complex128)
+        check(lib().hipmps_state_vector(self.h, out.view(np.float64).ctypes.data_as(_dp), out.size))
+        return out
+
+    def get_site(self, k: int) -> np.ndarray:
+        b = self.bonds()
+        out = np.zeros(b[k] * 2 * b[k + 1], dtype=np.complex128)
+        check(lib().hipmps_get_site(self.h, k, out.view(np.float64).ctypes.data_as(_dp), out.size))
+        return out.reshape(b[k], 2, b[k + 1])
+
+    def set_site(self, k: int, a: np.ndarray):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+        check(lib().hipmps_set_site(self.h, k, a.shape[0], a.shape[2], a.view(np.float64).ctypes.data_as(_dp)))
