@@ -134,6 +134,11 @@ int hipmps_expval_z_all(hipmps_state s, double* out_n);
 int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out);
 int hipmps_amplitudes(hipmps_state s, const int8_t* bits, int k, double* out_re_im);
 int hipmps_wavefunction_slice(hipmps_state s, const int* bits, double* out_re_im, size_t cap);
+/* <psi| (x)_k diag(d[2k], d[2k+1]) |psi>: Z strings, projectors (sequential sampling,
+ * getMeasureSample :2211-2364), occupation numbers */
+int hipmps_expval_diag(hipmps_state s, const double* diag_2n, double* out);
+/* deep copy of a state on the same engine (observable circuits in VQE mode) */
+int hipmps_state_clone(hipmps_state src, hipmps_state* out);
 int hipmps_state_vector(hipmps_state s, double* out_re_im, size_t cap); /* n <= 26 */
 /* copy one site tensor (chi_l x 2 x chi_r, complex128) to / from the host */
 int hipmps_get_site(hipmps_state s, int k, double* out, size_t cap);

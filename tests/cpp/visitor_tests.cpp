@@ -1,0 +1,439 @@
+// visitor_tests.cpp -- the hip-mps visitor through the TNQVM accelerator API.
+//
+// Cases follow the exatn-mps gtest suite (tnqvm/visitors/exatn-mps/tests/*.cpp) and the
+// closed-form checks of the other visitors' tests, run against "hip-mps":
+//   ./visitor_tests --cpu    IR / routing / option cases (no GPU)
+//   ./visitor_tests --gpu    circuit cases on the GPU
+// Exit status 0 when every selected case passes.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "TNQVM.hpp"
+
+using namespace xacc;
+
+namespace {
+int g_fail = 0, g_checks = 0;
+std::string g_case;
+
+#define EXPECT_NEAR(a, b, tol)                                                                  \
+  do {                                                                                          \
+    ++g_checks;                                                                                 \
+    double _a = (a), _b = (b);                                                                  \
+    if (!(std::fabs(_a - _b) <= (tol))) {                                                       \
+      ++g_fail;                                                                                 \
+      std::printf("  FAIL %s:%d [%s] %s = %.12g vs %.12g (tol %g)\n", __FILE__, __LINE__,        \
+                  g_case.c_str(), #a, _a, _b, (double)(tol));                                    \
+    }                                                                                           \
+  } while (0)
+#define EXPECT_TRUE(c)                                                                          \
+  do {                                                                                          \
+    ++g_checks;                                                                                 \
+    if (!(c)) {                                                                                 \
+      ++g_fail;                                                                                 \
+      std::printf("  FAIL %s:%d [%s] %s\n", __FILE__, __LINE__, g_case.c_str(), #c);           \
+    }                                                                                           \
+  } while (0)
+#define EXPECT_EQ(a, b) EXPECT_TRUE((a) == (b))
+
+struct Case {
+  const char* name;
+  bool gpu;
+  std::function<void()> fn;
+};
+std::vector<Case>& cases() {
+  static std::vector<Case> c;
+  return c;
+}
+struct Reg {
+  Reg(const char* n, bool gpu, std::function<void()> f) { cases().push_back({n, gpu, std::move(f)}); }
+};
+#define CASE(name, gpu) \
+  static void name();   \
+  static Reg reg_##name(#name, gpu, name); \
+  static void name()
+
+std::shared_ptr<CompositeInstruction> compile1(const std::string& src) {
+  return getCompiler("xasm")->compile(src)->getComposites()[0];
+}
+std::shared_ptr<Accelerator> mps(std::initializer_list<std::pair<std::string, std::any>> extra = {}) {
+  HeterogeneousMap m{{"tnqvm-visitor", std::string("hip-mps")}};
+  for (auto& kv : extra) {
+    if (kv.second.type() == typeid(int)) m.insert(kv.first, std::any_cast<int>(kv.second));
+    else if (kv.second.type() == typeid(double)) m.insert(kv.first, std::any_cast<double>(kv.second));
+    else if (kv.second.type() == typeid(bool)) m.insert(kv.first, std::any_cast<bool>(kv.second));
+    else if (kv.second.type() == typeid(std::string)) m.insert(kv.first, std::any_cast<std::string>(kv.second));
+    else if (kv.second.type() == typeid(const char*)) m.insert(kv.first, std::string(std::any_cast<const char*>(kv.second)));
+    else if (kv.second.type() == typeid(std::vector<int>)) m.insert(kv.first, std::any_cast<std::vector<int>>(kv.second));
+  }
+  return getAccelerator("tnqvm", m);
+}
+int count_swaps(const std::shared_ptr<CompositeInstruction>& f) {
+  int n = 0;
+  for (auto& i : f->getInstructions()) n += i->name() == "Swap";
+  return n;
+}
+
+// ------------------------------------------------------------------ CPU cases
+CASE(xasm_parse, false) {
+  auto f = compile1(R"(__qpu__ void t(qbit q, double th) {
+      H(q[0]);
+      for (int i = 0; i < 3; i++) { CNOT(q[i], q[i + 1]); }
+      U(q[1], th, -pi/2, pi/2);
+      Rx(q[2], 2*th);
+      Measure(q[0]);
+  })");
+  EXPECT_EQ(f->nInstructions(), (size_t)7);
+  auto g = (*f)({0.25});
+  EXPECT_NEAR(g->getInstruction(4)->getParameter(0).as<double>(), 0.25, 0);
+  EXPECT_NEAR(g->getInstruction(4)->getParameter(1).as<double>(), -constants::pi / 2, 1e-15);
+  EXPECT_NEAR(g->getInstruction(5)->getParameter(0).as<double>(), 0.5, 0);
+  EXPECT_EQ(g->getInstruction(3)->bits()[1], (size_t)3);
+}
+
+// NearestNeighborTransformTester.cpp:24-64
+CASE(lnn_transform_simple, false) {
+  auto f = compile1(R"(__qpu__ void test1(qbit q) {
+        X(q[0]);
+        CNOT(q[1], q[3]);
+        H(q[7]);
+        CNOT(q[6], q[7]);
+        CNOT(q[7], q[3]);
+        X(q[6]);
+        Swap(q[4], q[9]);
+        S(q[5]);
+    })");
+  auto opt = getService<IRTransformation>("lnn-transform");
+  const auto before = f->nInstructions();
+  const int sw0 = count_swaps(f);
+  opt->apply(f, nullptr);
+  const int added = count_swaps(f) - sw0;
+  EXPECT_EQ(f->nInstructions(), before + added);
+  for (auto& i : f->getInstructions())
+    if (i->bits().size() == 2) EXPECT_EQ(std::labs((long)i->bits()[0] - (long)i->bits()[1]), 1L);
+}
+
+// NearestNeighborTransformTester.cpp:67-100
+CASE(lnn_transform_max_distance, false) {
+  auto f = compile1(R"(__qpu__ void test2(qbit q) {
+        X(q[0]);
+        CNOT(q[1], q[9]);
+        H(q[7]);
+        CNOT(q[3], q[1]);
+    })");
+  auto opt = getService<IRTransformation>("lnn-transform");
+  const auto before = f->nInstructions();
+  const int sw0 = count_swaps(f);
+  HeterogeneousMap o;
+  o.insert("max-distance", 2);
+  opt->apply(f, nullptr, o);
+  const int added = count_swaps(f) - sw0;
+  EXPECT_EQ(f->nInstructions(), before + added);
+  auto last = f->getInstruction(f->nInstructions() - 1);
+  EXPECT_EQ(last->name(), std::string("CNOT"));
+  EXPECT_EQ(last->bits()[0], (size_t)3);
+  EXPECT_EQ(last->bits()[1], (size_t)1);
+  for (auto& i : f->getInstructions())
+    if (i->bits().size() == 2) EXPECT_TRUE(std::labs((long)i->bits()[0] - (long)i->bits()[1]) <= 2);
+}
+
+CASE(unknown_visitor_falls_back, false) {
+  auto acc = getAccelerator("tnqvm", {{"tnqvm-visitor", std::string("no-such-visitor")}});
+  EXPECT_EQ(std::static_pointer_cast<tnqvm::TNQVM>(acc)->getVisitorName(), std::string("hip-mps"));
+  bool threw = false;
+  try {
+    getAccelerator("tnqvm", {{"shots", 0}});
+  } catch (const std::exception&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);  // TNQVM.hpp:103-108
+}
+
+// ------------------------------------------------------------------ GPU cases
+// MpsGateTester.cpp:7-88
+CASE(gate_simple, true) {
+  const char* names[4] = {"1000", "0100", "0010", "0001"};
+  for (int k = 0; k < 4; ++k) {
+    auto f = compile1("__qpu__ void t(qbit q) { H(q[" + std::to_string(k) +
+                      "]); Measure(q[0]); Measure(q[1]); Measure(q[2]); Measure(q[3]); }");
+    auto acc = mps({{"shots", 10000}});
+    auto q = qalloc(4);
+    acc->execute(q, f);
+    EXPECT_NEAR(q->computeMeasurementProbability("0000"), 0.5, 0.05);
+    EXPECT_NEAR(q->computeMeasurementProbability(names[k]), 0.5, 0.05);
+  }
+}
+
+// MpsGateTester.cpp:90-218
+CASE(gate_two_qubit, true) {
+  struct T { const char* src; int n; const char* a; double pa; const char* b; double pb; };
+  std::vector<T> ts = {
+      {"H(q[0]); CNOT(q[0], q[1]); Measure(q[0]); Measure(q[1]);", 4, "00", 0.5, "11", 0.5},
+      {"H(q[1]); CNOT(q[1], q[0]); Measure(q[0]); Measure(q[1]);", 4, "00", 0.5, "11", 0.5},
+      {"H(q[2]); CNOT(q[2], q[3]); Measure(q[2]); Measure(q[3]);", 4, "00", 0.5, "11", 0.5},
+      {"H(q[3]); CNOT(q[3], q[2]); Measure(q[2]); Measure(q[3]);", 4, "00", 0.5, "11", 0.5},
+      {"X(q[0]); CNOT(q[0], q[1]); CNOT(q[1], q[2]); CNOT(q[2], q[3]); Measure(q[0]); Measure(q[1]); Measure(q[2]); Measure(q[3]);",
+       4, "1111", 1.0, "0000", 0.0},
+  };
+  for (auto& t : ts) {
+    g_case = t.src;
+    auto acc = mps({{"shots", 10000}});
+    auto q = qalloc(t.n);
+    acc->execute(q, compile1(std::string("__qpu__ void f(qbit q) {") + t.src + "}"));
+    EXPECT_NEAR(q->computeMeasurementProbability(t.a), t.pa, 0.05);
+    EXPECT_NEAR(q->computeMeasurementProbability(t.b), t.pb, 0.05);
+  }
+}
+
+// MpsGateTester.cpp:220-240 (non-neighbouring CNOT, routed by the visitor)
+CASE(gate_distance, true) {
+  auto acc = mps({{"shots", 10000}});
+  auto q = qalloc(4);
+  acc->execute(q, compile1("__qpu__ void f(qbit q) { X(q[0]); CNOT(q[0], q[3]); Measure(q[0]); Measure(q[1]); Measure(q[2]); Measure(q[3]); }"));
+  EXPECT_NEAR(q->computeMeasurementProbability("1001"), 1.0, 0.01);
+  // exact amplitude of the routed result
+  auto acc2 = mps({{"bitstring", std::vector<int>{1, 0, 0, 1}}});
+  auto q2 = qalloc(4);
+  acc2->execute(q2, compile1("__qpu__ void f(qbit q) { X(q[0]); CNOT(q[0], q[3]); }"));
+  EXPECT_NEAR((*q2)["amplitude-real"].as<double>(), 1.0, 1e-12);
+}
+
+// MpsGateTester.cpp:242-310: 2- and 1-qubit registers
+CASE(gate_small_registers, true) {
+  {
+    auto acc = mps({{"shots", 10000}});
+    auto q = qalloc(2);
+    acc->execute(q, compile1("__qpu__ void f(qbit q) { X(q[0]); CNOT(q[0], q[1]); Measure(q[0]); Measure(q[1]); }"));
+    EXPECT_NEAR(q->computeMeasurementProbability("11"), 1.0, 0.01);
+  }
+  {
+    auto acc = mps({{"shots", 10000}});
+    auto q = qalloc(2);
+    acc->execute(q, compile1("__qpu__ void f(qbit q) { X(q[1]); CNOT(q[1], q[0]); Measure(q[0]); Measure(q[1]); }"));
+    EXPECT_NEAR(q->computeMeasurementProbability("11"), 1.0, 0.01);
+  }
+  {
+    auto acc = mps({{"shots", 10000}});
+    auto q = qalloc(1);
+    acc->execute(q, compile1("__qpu__ void f(qbit q) { H(q[0]); Z(q[0]); H(q[0]); Measure(q[0]); }"));
+    EXPECT_NEAR(q->computeMeasurementProbability("1"), 1.0, 0.01);
+  }
+}
+
+// MpsGateTester.cpp:312-357
+CASE(gate_iswap_fsim, true) {
+  {
+    auto acc = mps({{"shots", 100}});
+    auto q = qalloc(3);
+    acc->execute(q, compile1("__qpu__ void f(qbit q) { X(q[0]); iSwap(q[0], q[1]); Measure(q[0]); Measure(q[1]); Measure(q[2]); }"));
+    EXPECT_NEAR(q->computeMeasurementProbability("010"), 1.0, 0.01);
+  }
+  auto prog = compile1("__qpu__ void f(qbit q, double x, double y) { X(q[0]); fSim(q[0], q[1], x, y); Measure(q[0]); Measure(q[1]); }");
+  for (double a : linspace(-constants::pi, constants::pi, 10)) {
+    auto acc = mps({{"shots", 1000}});
+    auto q = qalloc(2);
+    acc->execute(q, (*prog)({a, 0.0}));
+    EXPECT_NEAR(q->computeMeasurementProbability("01"), std::sin(a) * std::sin(a), 0.1);
+    EXPECT_NEAR(q->computeMeasurementProbability("10"), 1.0 - std::sin(a) * std::sin(a), 0.1);
+    // exact: <Z0 Z1> = -1 for every angle (one excitation)
+    auto acc2 = mps();
+    auto q2 = qalloc(2);
+    acc2->execute(q2, (*prog)({a, 0.3}));
+    EXPECT_NEAR(q2->getExpectationValueZ(), -1.0, 1e-12);
+  }
+}
+
+// MpsGateTester.cpp:359-407 (table over all angles, exact exp-val-z instead of shots)
+CASE(deuteron, true) {
+  const std::vector<double> expected{0.0,      -0.324699, -0.614213, -0.837166, -0.9694,  -0.996584, -0.915773,
+                                     -0.735724, -0.475947, -0.164595, 0.164595,  0.475947, 0.735724,  0.915773,
+                                     0.996584,  0.9694,    0.837166,  0.614213,  0.324699, 0.0};
+  auto prog = compile1(R"(__qpu__ void ansatz(qbit q, double t) {
+      X(q[0]); Ry(q[1], t); CX(q[1], q[0]); H(q[0]); H(q[1]); Measure(q[0]); Measure(q[1]); })");
+  auto angles = linspace(-constants::pi, constants::pi, 20);
+  for (size_t i = 0; i < angles.size(); ++i) {
+    auto acc = mps();
+    auto q = qalloc(2);
+    acc->execute(q, (*prog)({angles[i]}));
+    EXPECT_NEAR(q->getExpectationValueZ(), expected[i], 1e-6);
+  }
+}
+
+// MpsGateTester.cpp:409-499
+CASE(grover, true) {
+  auto acc = mps({{"shots", 10000}});
+  auto q = qalloc(3);
+  acc->execute(q, compile1(R"(__qpu__ void g(qbit q) {
+      H(q[0]); H(q[1]); H(q[2]); X(q[0]); H(q[2]); H(q[2]); CNOT(q[1], q[2]); Tdg(q[2]); CNOT(q[0], q[2]);
+      T(q[2]); CNOT(q[1], q[2]); Tdg(q[2]); CNOT(q[0], q[2]); T(q[2]); H(q[2]); T(q[1]); CNOT(q[0], q[1]);
+      T(q[0]); Tdg(q[1]); CNOT(q[0], q[1]); X(q[0]); H(q[2]); H(q[0]); H(q[1]); H(q[2]); X(q[0]); X(q[1]);
+      X(q[2]); H(q[2]); H(q[2]); CNOT(q[1], q[2]); Tdg(q[2]); CNOT(q[0], q[2]); T(q[2]); CNOT(q[1], q[2]);
+      Tdg(q[2]); CNOT(q[0], q[2]); T(q[2]); H(q[2]); T(q[1]); CNOT(q[0], q[1]); T(q[0]); Tdg(q[1]);
+      CNOT(q[0], q[1]); H(q[2]); X(q[0]); X(q[1]); X(q[2]); H(q[0]); H(q[1]); H(q[2]);
+      Measure(q[2]); Measure(q[1]); Measure(q[0]); })"));
+  EXPECT_TRUE(q->computeMeasurementProbability("110") > 0.5);
+}
+
+// MpsMeasurementTester.cpp:7-35: 35-qubit GHZ sampled through the MPS (n >= 20 path)
+CASE(measure_ghz35, true) {
+  auto acc = mps({{"shots", 64}});
+  auto q = qalloc(35);
+  acc->execute(q, compile1(R"(__qpu__ void t(qbit q) {
+      H(q[0]);
+      for (int i = 0; i < 34; i++) { CNOT(q[i], q[i + 1]); }
+      Measure(q[5]); Measure(q[3]); Measure(q[7]); Measure(q[34]); })"));
+  EXPECT_NEAR(q->computeMeasurementProbability("0000") + q->computeMeasurementProbability("1111"), 1.0, 1e-12);
+}
+
+// MpsMeasurementTester.cpp:37-66
+CASE(measure_seed_determinism, true) {
+  std::vector<std::map<std::string, int>> results;
+  for (int i = 0; i < 20; ++i) {
+    auto acc = mps({{"shots", 8192}, {"seed", 123}});
+    auto q = qalloc(4);
+    acc->execute(q, compile1(R"(__qpu__ void t(qbit q) {
+        H(q[0]); for (int i = 0; i < 3; i++) { CNOT(q[i], q[i + 1]); }
+        Measure(q[0]); Measure(q[1]); Measure(q[2]); Measure(q[3]); })"));
+    results.push_back(q->getMeasurementCounts());
+  }
+  for (auto& r : results) EXPECT_TRUE(r == results[0]);
+}
+
+// NumericalTesterCheckNorm.cpp:15-62, with a seeded layer circuit instead of "rcs"
+CASE(norm_check, true) {
+  std::mt19937_64 rng(7);
+  std::uniform_int_distribution<int> pick(0, 7);
+  std::uniform_real_distribution<double> ang(-constants::pi, constants::pi);
+  auto prov = getIRProvider("quantum");
+  auto c = prov->createComposite("rcs");
+  const int n = 10;
+  for (int layer = 0; layer < 15; ++layer) {
+    for (size_t q = 0; q < (size_t)n; ++q) {
+      const char* g1[8] = {"H", "X", "Y", "Z", "T", "Rx", "Ry", "Rz"};
+      int k = pick(rng);
+      if (k < 5) c->addInstruction(prov->createInstruction(g1[k], {q}));
+      else c->addInstruction(prov->createInstruction(g1[k], {q}, {InstructionParameter(ang(rng))}));
+    }
+    for (size_t q = 0; q + 1 < (size_t)n; ++q) c->addInstruction(prov->createInstruction("CNOT", {q, q + 1}));
+  }
+  for (double cut : {0.0, 1e-16}) {
+    auto acc = cut == 0.0 ? mps({{"shots", 1}}) : mps({{"shots", 1}, {"svd-cutoff", 1e-16}});
+    auto q = qalloc(n);
+    acc->execute(q, c);
+    EXPECT_NEAR((*q)["norm"].as<double>(), 1.0, 1e-10);
+  }
+}
+
+// ITensorMPSVisitorTester.cpp:310-392 closed forms, exact
+CASE(rotation_kats, true) {
+  auto rx = compile1("__qpu__ void r(qbit q, double t) { Rx(q[0], t); Measure(q[0]); }");
+  auto u3 = compile1("__qpu__ void r(qbit q, double t) { U(q[0], t, -pi/2, pi/2); Measure(q[0]); }");
+  auto cp = compile1("__qpu__ void r(qbit q, double t) { X(q[0]); H(q[1]); CPhase(q[0], q[1], t); H(q[1]); Measure(q[1]); }");
+  for (double a : linspace(-constants::pi, constants::pi, 20)) {
+    const double want = 1.0 - 2.0 * std::sin(a / 2) * std::sin(a / 2);
+    for (auto& p : {rx, u3}) {
+      auto q = qalloc(1);
+      mps()->execute(q, (*p)({a}));
+      EXPECT_NEAR(q->getExpectationValueZ(), want, 1e-12);
+    }
+    auto q = qalloc(2);
+    mps()->execute(q, (*cp)({a}));
+    EXPECT_NEAR(q->getExpectationValueZ(), want, 1e-12);
+  }
+}
+
+// amplitudes (ITensorMPSVisitorTester.cpp:522-551; bitstring_amplitude example)
+CASE(amplitudes, true) {
+  auto q = qalloc(3);
+  mps({{"bitstring", std::vector<int>{1, 1, 1}}})->execute(q, compile1("__qpu__ void g(qbit q) { H(q[0]); Z(q[0]); CX(q[0], q[1]); CX(q[1], q[2]); }"));
+  EXPECT_NEAR((*q)["amplitude-real"].as<double>(), -1.0 / std::sqrt(2.0), 1e-12);
+  EXPECT_NEAR((*q)["amplitude-imag"].as<double>(), 0.0, 1e-12);
+  // slice over all qubits of an n-qubit GHZ: 1/sqrt2 at 0 and 2^n - 1 (ExaTnGenTester.cpp:188-218)
+  auto q2 = qalloc(5);
+  mps({{"bitstring", std::vector<int>{-1, -1, -1, -1, -1}}})->execute(
+      q2, compile1("__qpu__ void g(qbit q) { H(q[0]); for (int i = 0; i < 4; i++) { CX(q[i], q[i + 1]); } }"));
+  auto re = (*q2)["amplitude-real-vec"].as<std::vector<double>>();
+  EXPECT_EQ(re.size(), (size_t)32);
+  EXPECT_NEAR(re[0], 1.0 / std::sqrt(2.0), 1e-12);
+  EXPECT_NEAR(re[31], 1.0 / std::sqrt(2.0), 1e-12);
+}
+
+// VQE mode: the shared ansatz once, observables on that state == each kernel alone
+CASE(vqe_mode, true) {
+  auto prov = getIRProvider("quantum");
+  auto build = [&](const std::string& basis) {
+    auto c = prov->createComposite("k_" + basis);
+    c->addInstruction(prov->createInstruction("X", {0}));
+    c->addInstruction(prov->createInstruction("Ry", {1}, {InstructionParameter(0.59)}));
+    c->addInstruction(prov->createInstruction("CNOT", {1, 0}));
+    c->addInstruction(prov->createInstruction("Rz", {2}, {InstructionParameter(0.3)}));
+    c->addInstruction(prov->createInstruction("CNOT", {1, 2}));
+    if (basis == "XX") {
+      c->addInstruction(prov->createInstruction("H", {0}));
+      c->addInstruction(prov->createInstruction("H", {1}));
+    }
+    if (basis == "YY") {
+      c->addInstruction(prov->createInstruction("Rx", {0}, {InstructionParameter(constants::pi / 2)}));
+      c->addInstruction(prov->createInstruction("Rx", {1}, {InstructionParameter(constants::pi / 2)}));
+    }
+    if (basis == "CZ") c->addInstruction(prov->createInstruction("CNOT", {0, 2}));
+    c->addInstruction(prov->createInstruction("Measure", {0}));
+    c->addInstruction(prov->createInstruction("Measure", {1}));
+    return c;
+  };
+  std::vector<std::shared_ptr<CompositeInstruction>> ks = {build("ZZ"), build("XX"), build("YY"), build("CZ")};
+  auto qv = qalloc(3);
+  mps({{"vqe-mode", true}})->execute(qv, ks);
+  auto qn = qalloc(3);
+  mps({{"vqe-mode", false}})->execute(qn, ks);
+  auto cv = qv->getChildren(), cn = qn->getChildren();
+  EXPECT_EQ(cv.size(), (size_t)4);
+  for (size_t i = 0; i < cv.size() && i < cn.size(); ++i)
+    EXPECT_NEAR(cv[i]->getExpectationValueZ(), cn[i]->getExpectationValueZ(), 1e-12);
+}
+
+CASE(accelerator_state, true) {
+  auto acc = std::static_pointer_cast<tnqvm::TNQVM>(mps());
+  auto sv = acc->getAcceleratorState(compile1("__qpu__ void b(qbit q) { H(q[0]); CNOT(q[0], q[1]); }"));
+  EXPECT_EQ(sv.size(), (size_t)4);
+  EXPECT_NEAR(std::abs(sv[0]), 1.0 / std::sqrt(2.0), 1e-14);
+  EXPECT_NEAR(std::abs(sv[3]), 1.0 / std::sqrt(2.0), 1e-14);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  bool want_gpu = false, want_cpu = false;
+  const char* filter = nullptr;
+  for (int i = 1; i < argc; ++i) {
+    if (!strcmp(argv[i], "--gpu")) want_gpu = true;
+    else if (!strcmp(argv[i], "--cpu")) want_cpu = true;
+    else filter = argv[i];
+  }
+  if (!want_gpu && !want_cpu) want_cpu = true;
+  int ran = 0;
+  for (auto& c : cases()) {
+    if ((c.gpu && !want_gpu) || (!c.gpu && !want_cpu)) continue;
+    if (filter && !strstr(c.name, filter)) continue;
+    g_case = c.name;
+    const int before = g_fail;
+    std::printf("[case] %s\n", c.name);
+    std::fflush(stdout);
+    try {
+      c.fn();
+    } catch (const std::exception& e) {
+      ++g_fail;
+      std::printf("  FAIL [%s] exception: %s\n", c.name, e.what());
+    }
+    std::printf("[%s] %s\n", g_fail == before ? "ok" : "FAILED", c.name);
+    std::fflush(stdout);
+    ++ran;
+  }
+  std::printf("%d cases, %d checks, %d failures\n", ran, g_checks, g_fail);
+  return g_fail ? 1 : 0;
+}

@@ -33,6 +33,7 @@ EXPORTS = [
     "hipmps_amplitudes", "hipmps_wavefunction_slice", "hipmps_state_vector", "hipmps_get_site",
     "hipmps_set_site", "hipmps_run_batch", "hipmps_serve_submit", "hipmps_serve_step",
     "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing",
+    "hipmps_expval_diag", "hipmps_state_clone",
 ]
 
 
@@ -127,6 +128,8 @@ def lib():
         "hipmps_serve_pending": ([vp, pint, pint], ip),
         "hipmps_engine_profile": ([vp, ctypes.POINTER(Profile), ip], ip),
         "hipmps_engine_set_timing": ([vp, ip], ip),
+        "hipmps_expval_diag": ([vp, dp, dp], ip),
+        "hipmps_state_clone": ([vp, ctypes.POINTER(vp)], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -156,7 +156,7 @@ struct hipmps_engine_s {
   size_t used = 0;  // bytes of site storage held by states
   int max_sweeps = 40;
   double tol = 1e-13;
-  int inner_sweeps = 10;
+  int inner_sweeps = 1;
   DevBuf ws;     // per-wave arena
   DevBuf desc;   // task descriptors
   DevBuf small;  // counters / K / stats
@@ -672,7 +672,8 @@ struct EnvJob {
   const double2* Ein;
   double2* X;
   double2* Eout;
-  int z;
+  int z;              // 1: apply diag(d0, d1) on the physical index of this site
+  double d0 = 1.0, d1 = -1.0;
 };
 
 // left step: E'(cr x cr) = A^H D (E A), D = diag(+-1) on the physical index when z
@@ -700,6 +701,8 @@ int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
     b.C = j.Eout; b.ldc = j.cr;
     b.M = j.cr; b.N = j.cr; b.K = 2 * j.cl;
     b.sign_div = j.z ? 1 : 0;
+    b.kd0 = j.d0;
+    b.kd1 = j.d1;
     b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
     b.tile_begin = t2; t2 += b.tiles_m * b.tiles_n;
   }
@@ -736,6 +739,8 @@ int env_right(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
     b.C = j.Eout; b.ldc = j.cl;
     b.M = j.cl; b.N = j.cl; b.K = 2 * j.cr;
     b.sign_div = j.z ? j.cr : 0;
+    b.kd0 = j.d0;
+    b.kd1 = j.d1;
     b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
     b.tile_begin = t2; t2 += b.tiles_m * b.tiles_n;
   }
@@ -776,17 +781,33 @@ EnvLayout env_layout(const hipmps_state_s* s, size_t extra_mats, size_t extra_el
   return L;
 }
 
-int compute_left_envs(hipmps_state_s* s, const EnvLayout& L, const std::vector<int>& zmask) {
+// diag: empty (identity everywhere) or 2n values diag(d_k0, d_k1) per site
+int compute_left_envs_diag(hipmps_state_s* s, const EnvLayout& L, const std::vector<double>& diag) {
   hipmps_engine E = s->e;
   char* base = (char*)E->env.p;
   HIP_TRY(launch_fill_identity1((double2*)(base + L.offL[0]), E->stream));
   for (int k = 0; k < s->n; ++k) {
     std::vector<EnvJob> jobs(1);
     jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offL[k]),
-               (double2*)(base + L.offX), (double2*)(base + L.offL[k + 1]), zmask.empty() ? 0 : zmask[k]};
+               (double2*)(base + L.offX), (double2*)(base + L.offL[k + 1]), 0};
+    if (!diag.empty() && !(diag[2 * k] == 1.0 && diag[2 * k + 1] == 1.0)) {
+      jobs[0].z = 1;
+      jobs[0].d0 = diag[2 * k];
+      jobs[0].d1 = diag[2 * k + 1];
+    }
     RC_TRY(env_left(E, jobs, s->fp32));
   }
   return 0;
+}
+
+int compute_left_envs(hipmps_state_s* s, const EnvLayout& L, const std::vector<int>& zmask) {
+  std::vector<double> diag;
+  if (!zmask.empty()) {
+    diag.assign(2 * s->n, 1.0);
+    for (int k = 0; k < s->n; ++k)
+      if (zmask[k]) diag[2 * k + 1] = -1.0;
+  }
+  return compute_left_envs_diag(s, L, diag);
 }
 
 int compute_right_envs(hipmps_state_s* s, const EnvLayout& L) {
@@ -947,6 +968,17 @@ int measure_parity(hipmps_state_s* s, const std::vector<int>& zmask, double* out
   EnvLayout L = env_layout(s, 1, 0);
   RC_TRY(E->env.ensure(L.total));
   RC_TRY(compute_left_envs(s, L, zmask));
+  double2 v;
+  RC_TRY(read_scalar(E, (char*)E->env.p + L.offL[s->n], &v));
+  *out = v.x;
+  return 0;
+}
+
+int measure_diag(hipmps_state_s* s, const std::vector<double>& diag, double* out) {
+  hipmps_engine E = s->e;
+  EnvLayout L = env_layout(s, 1, 0);
+  RC_TRY(E->env.ensure(L.total));
+  RC_TRY(compute_left_envs_diag(s, L, diag));
   double2 v;
   RC_TRY(read_scalar(E, (char*)E->env.p + L.offL[s->n], &v));
   *out = v.x;
@@ -1208,6 +1240,46 @@ int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out) {
   RC_TRY(state_flush(s));
   if (k == 0) return 0;
   return measure_zz(s, pairs, k, out);
+}
+
+int hipmps_expval_diag(hipmps_state s, const double* diag, double* out) {
+  if (!s || !diag || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  std::vector<double> d(diag, diag + 2 * s->n);
+  return measure_diag(s, d, out);
+}
+
+int hipmps_state_clone(hipmps_state src, hipmps_state* out) {
+  if (!src || !out) return fail(HIPMPS_E_ARG, "null argument");
+  *out = nullptr;
+  RC_TRY(state_flush(src));
+  hipmps_state dst = nullptr;
+  RC_TRY(hipmps_state_create(src->e, src->n, &src->t, &dst));
+  for (int k = 0; k < src->n; ++k) {
+    size_t n = (size_t)src->chi[k] * 2 * src->chi[k + 1];
+    int rc = ensure_site(dst, k, n, false, nullptr);
+    if (rc) {
+      hipmps_state_destroy(dst);
+      return rc;
+    }
+    if (hipMemcpyAsync(dst->sites[k].p, src->sites[k].p, n * src->es, hipMemcpyDeviceToDevice, src->e->stream) !=
+        hipSuccess) {
+      hipmps_state_destroy(dst);
+      return fail(HIPMPS_E_DEVICE, "site copy failed");
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(src->e->stream));
+  dst->chi = src->chi;
+  dst->discarded = src->discarded;
+  dst->fid = src->fid;
+  dst->maxb = src->maxb;
+  dst->n2q = src->n2q;
+  dst->n1q = src->n1q;
+  dst->sweeps_max = src->sweeps_max;
+  dst->sweeps_total = src->sweeps_total;
+  dst->status = src->status;
+  *out = dst;
+  return 0;
 }
 
 int hipmps_amplitudes(hipmps_state s, const int8_t* bits, int k, double* out) {

@@ -80,9 +80,10 @@ struct GemmTask {
   const int* gatherB;     // T/H: storage row of B for output column j
   const double* rowscale; // per output row
   const double* colscale; // per output column
-  int sign_div;           // >0: a(i,k) *= (-1)^((k / sign_div) & 1)
+  int sign_div;           // >0: a(i,k) *= ((k / sign_div) & 1) ? kd1 : kd0
   int tile_begin, tiles_m, tiles_n;
   int accumulate;         // C += result
+  double kd0, kd1;        // diagonal operator on the physical index (Z: 1, -1)
 };
 
 // Chain step for amplitudes / slices: rows of partial contractions v[row][l].

@@ -276,9 +276,21 @@ __device__ __forceinline__ void jacobi_rotation(double alpha, double beta, doubl
   se = make_double2(s * gamma.x / g, s * gamma.y / g);
 }
 
+// FP64 MFMA v_mfma_f64_16x16x4f64 operand layout on gfx950 (measured):
+//   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16*(i%4),
+//   accumulator register i/4.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mfma_d4 mfma64(double a, double b, mfma_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Column chunk of the 2*JB rows staged in LDS; 32 columns per chunk.
+constexpr int JC = 32;
+
 __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int ntasks, int round, double tol,
                                                       int inner_sweeps) {
-  __shared__ double2 sY[JP][33];
+  __shared__ double2 sY[JP][JC + 1];
   __shared__ double2 sG[JP][JP + 1];
   __shared__ double2 sW[JP][JP + 1];
   __shared__ double s_c[JP / 2];
@@ -292,37 +304,53 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
   const int nb = T.nb;
   if (pi >= nb / 2 || round >= nb - 1) return;
   const int ba = circle_slot(pi, round, nb), bb = circle_slot(nb - 1 - pi, round, nb);
-  const int C = T.c;
+  const int C = T.c, CT = T.c_tot;
   const size_t ld = T.ldy;
   double2* Y = T.Y;
   const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int l16 = lane & 15, k4 = lane >> 4;
   auto grow = [&](int j) { return j < JB ? ba * JB + j : bb * JB + (j - JB); };
 
-  // ---- Gram ----
-  const int gi = tid / 8, gj0 = (tid % 8) * 4;
-  double2 acc[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = make_double2(0, 0);
-  for (int c0 = 0; c0 < C; c0 += 32) {
+  // chunk loader: 32 rows x 32 columns, 4 elements per thread, coalesced along columns
+  double2 pre[4];
+  auto load_chunk = [&](int c0, int climit) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       int e = tid + 256 * q;
-      int row = e / 32, col = e % 32;
-      double2 v = make_double2(0, 0);
-      if (c0 + col < C) v = Y[(size_t)grow(row) * ld + c0 + col];
-      sY[row][col] = v;
+      int row = e >> 5, col = e & 31;
+      pre[q] = (c0 + col < climit) ? Y[(size_t)grow(row) * ld + c0 + col] : make_double2(0, 0);
     }
-    __syncthreads();
-#pragma unroll 8
-    for (int cc = 0; cc < 32; ++cc) {
-      double2 yi = sY[gi][cc];
+  };
+  auto store_chunk = [&]() {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] = cfmac(yi, sY[gj0 + q][cc], acc[q]);
+    for (int q = 0; q < 4; ++q) {
+      int e = tid + 256 * q;
+      sY[e >> 5][e & 31] = pre[q];
+    }
+  };
+
+  // ---- Gram  G[i][j] = sum_c conj(Y[i][c]) Y[j][c]  (MFMA, wave w -> tile (w>>1, w&1)) ----
+  const int ti = wave >> 1, tj = wave & 1;
+  mfma_d4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
+  load_chunk(0, C);
+  for (int c0 = 0; c0 < C; c0 += JC) {
+    store_chunk();
+    __syncthreads();
+    if (c0 + JC < C) load_chunk(c0 + JC, C);
+#pragma unroll
+    for (int kk = 0; kk < JC / 4; ++kk) {
+      const double2 a = sY[ti * 16 + l16][kk * 4 + k4];
+      const double2 b = sY[tj * 16 + l16][kk * 4 + k4];
+      gr = mfma64(a.x, b.x, gr);
+      gr = mfma64(a.y, b.y, gr);
+      gi = mfma64(a.x, b.y, gi);
+      gi = mfma64(-a.y, b.x, gi);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) sG[gi][gj0 + q] = acc[q];
+  for (int v = 0; v < 4; ++v) sG[ti * 16 + 4 * v + k4][tj * 16 + l16] = make_double2(gr[v], gi[v]);
   __syncthreads();
 
   // ---- convergence test of this block pair ----
@@ -347,7 +375,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
   if (tid == 0) atomicAdd(T.visited, 1);
   if (max_offdiag() <= tol) return;
 
-  // ---- eigen-decomposition of the Gram block (cyclic Jacobi, FP64) ----
+  // ---- eigen-decomposition of the Gram block (cyclic two-sided Jacobi, FP64) ----
   for (int e = tid; e < JP * JP; e += 256) sW[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
   __syncthreads();
   for (int sw = 0; sw < inner_sweeps; ++sw) {
@@ -392,33 +420,37 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
       }
       __syncthreads();
     }
-    if (max_offdiag() <= tol * 0.25) break;
+    if (sw + 1 < inner_sweeps && max_offdiag() <= tol * 0.25) break;
   }
 
-  // ---- apply: Y'[i][c] = sum_j W[j][i] Y[j][c], over data and rotation columns ----
-  const int CT = T.c_tot;
-  const int ac = tid % 32, ai0 = (tid / 32) * 4;
-  for (int c0 = 0; c0 < CT; c0 += 32) {
+  // ---- apply  Y'[i][c] = sum_j W[j][i] Y[j][c]  over data and rotation columns (MFMA) ----
+  // wave w -> output rows (w & 1)*16.., output column tile (w >> 1) of each 32-column chunk
+  const int ri = wave & 1, ct = wave >> 1;
+  double ar[JP / 4], ai[JP / 4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int e = tid + 256 * q;
-      int row = e / 32, col = e % 32;
-      double2 v = make_double2(0, 0);
-      if (c0 + col < CT) v = Y[(size_t)grow(row) * ld + c0 + col];
-      sY[row][col] = v;
-    }
+  for (int kk = 0; kk < JP / 4; ++kk) {
+    double2 w = sW[kk * 4 + k4][ri * 16 + l16];  // A[i][k] = W[k][i]
+    ar[kk] = w.x;
+    ai[kk] = w.y;
+  }
+  load_chunk(0, CT);
+  for (int c0 = 0; c0 < CT; c0 += JC) {
+    store_chunk();
     __syncthreads();
-    double2 out[4];
+    if (c0 + JC < CT) load_chunk(c0 + JC, CT);
+    mfma_d4 dr = {0, 0, 0, 0}, di = {0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[q] = make_double2(0, 0);
-    for (int j = 0; j < JP; ++j) {
-      double2 y = sY[j][ac];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) out[q] = cfma(sW[j][ai0 + q], y, out[q]);
+    for (int kk = 0; kk < JP / 4; ++kk) {
+      const double2 b = sY[kk * 4 + k4][ct * 16 + l16];
+      dr = mfma64(ar[kk], b.x, dr);
+      dr = mfma64(-ai[kk], b.y, dr);
+      di = mfma64(ar[kk], b.y, di);
+      di = mfma64(ai[kk], b.x, di);
     }
-    if (c0 + ac < CT) {
+    const int col = c0 + ct * 16 + l16;
+    if (col < CT) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) Y[(size_t)grow(ai0 + q) * ld + c0 + ac] = out[q];
+      for (int v = 0; v < 4; ++v) Y[(size_t)grow(ri * 16 + 4 * v + k4) * ld + col] = make_double2(dr[v], di[v]);
     }
     __syncthreads();
   }
@@ -622,7 +654,8 @@ hipError_t launch_gather(const SplitTask* d_split, int ntasks, int max_k, int ma
 //   a(i,k): N  A[gA(i)*lda + k]   C conj(A[gA(i)*lda + k])
 //           H  conj(A[k*lda + i]) T A[k*lda + i]
 //   b(k,j): N  B[k*ldb + j]       T B[gB(j)*ldb + k]   H conj(B[gB(j)*ldb + k])
-// optional sign (-1)^((k / sign_div) & 1) on a(i,k) (a diagonal Z inserted on the k index).
+// optional diagonal operator diag(kd0, kd1) on a(i,k), selected by the physical index
+// (k / sign_div) & 1 (Z, projectors, ... inserted on the contracted physical leg).
 constexpr int GM = 32, GN = 32, GK = 16;
 
 template <typename TA, typename TB, typename TC, int OPA, int OPB>
@@ -667,7 +700,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmTask* tasks, int ntasks)
           v = ld(A, (size_t)gk * T.lda + gi);
         }
         if (OPA == 1 || OPA == 2) v = cconj(v);
-        if (T.sign_div > 0 && ((gk / T.sign_div) & 1)) v = cscale(v, -1.0);
+        if (T.sign_div > 0) v = cscale(v, ((gk / T.sign_div) & 1) ? T.kd1 : T.kd0);
       }
       sA[i][k] = v;
     }
