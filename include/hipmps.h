@@ -178,10 +178,13 @@ int hipmps_serve_pending(hipmps_engine e, int* queued, int* running);
 /* ---- measurement hooks for the bench ------------------------------------------------ */
 typedef struct {
   double ms_theta, ms_jacobi, ms_split, ms_other;
+  double ms_jacobi_rounds;  /* HIP-event time of the back-to-back k_jacobi_round launches */
   long long jacobi_launches;
   long long jacobi_pairs_rotated;
   long long jacobi_pairs_visited;
   double jacobi_flops; /* flops executed by the Jacobi block-rotation kernel */
+  long long waves;      /* batched waves run */
+  long long wave_tasks; /* 2-qubit gates applied in those waves */
 } hipmps_profile_t;
 int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset);
 int hipmps_engine_set_timing(hipmps_engine e, int enabled);

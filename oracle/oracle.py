@@ -60,6 +60,7 @@ def lib():
         L.orc_mps_site_size.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_mps_site_size.restype = ctypes.c_size_t
         L.orc_mps_get_site.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_mps_set_site.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_mps_state_vector.argtypes = [ctypes.c_void_p, _dp]
         L.orc_mps_expval_z_parity.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int]
         L.orc_mps_expval_z_parity.restype = ctypes.c_double
@@ -207,6 +208,10 @@ class Mps:
         out = np.zeros(b[k] * 2 * b[k + 1], dtype=np.complex128)
         lib().orc_mps_get_site(self.h, k, _d(out.view(np.float64)))
         return out.reshape(b[k], 2, b[k + 1])
+
+    def set_site(self, k: int, a: np.ndarray):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+        lib().orc_mps_set_site(self.h, k, a.shape[0], a.shape[2], _d(a.view(np.float64)))
 
     def state_vector(self) -> np.ndarray:
         out = np.zeros(1 << self.n, dtype=np.complex128)

@@ -79,9 +79,10 @@ class Result(ctypes.Structure):
 
 class Profile(ctypes.Structure):
     _fields_ = [("ms_theta", ctypes.c_double), ("ms_jacobi", ctypes.c_double), ("ms_split", ctypes.c_double),
-                ("ms_other", ctypes.c_double), ("jacobi_launches", ctypes.c_longlong),
-                ("jacobi_pairs_rotated", ctypes.c_longlong), ("jacobi_pairs_visited", ctypes.c_longlong),
-                ("jacobi_flops", ctypes.c_double)]
+                ("ms_other", ctypes.c_double), ("ms_jacobi_rounds", ctypes.c_double),
+                ("jacobi_launches", ctypes.c_longlong), ("jacobi_pairs_rotated", ctypes.c_longlong),
+                ("jacobi_pairs_visited", ctypes.c_longlong), ("jacobi_flops", ctypes.c_double),
+                ("waves", ctypes.c_longlong), ("wave_tasks", ctypes.c_longlong)]
 
 
 _lib = None

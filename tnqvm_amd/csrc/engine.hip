@@ -166,7 +166,7 @@ struct hipmps_engine_s {
   std::mutex mu;
   bool timing = false;
   hipmps_profile_t prof{};
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // serving
   uint64_t next_ticket = 1;
   std::deque<PendingCircuit*> queue;
@@ -482,11 +482,18 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     // reuse the JacTask region of the device descriptor block for the active list
     HIP_TRY(hipMemcpyAsync(djt, hja, nact * sizeof(JacTask), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * N * sizeof(int), st));
+    if (E->timing) HIP_TRY(hipEventRecord(E->ev[4], st));
     for (int round = 0; round < max_rounds; ++round)
       HIP_TRY(launch_jacobi_round(djt, nact, pairs, round, E->tol, E->inner_sweeps, st));
+    if (E->timing) HIP_TRY(hipEventRecord(E->ev[5], st));
     E->prof.jacobi_launches += max_rounds;
     HIP_TRY(hipMemcpyAsync(hcounts, d_counts, 2 * N * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (E->timing) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, E->ev[4], E->ev[5]);
+      E->prof.ms_jacobi_rounds += ms;
+    }
     std::vector<int> next;
     for (int a = 0; a < nact; ++a) {
       int q = active[a];
@@ -533,6 +540,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     E->prof.ms_jacobi += b;
     E->prof.ms_split += c;
   }
+  E->prof.waves += 1;
+  E->prof.wave_tasks += N;
   for (int q = 0; q < N; ++q) {
     auto& it = items[q];
     hipmps_state_s* s = it.s;
