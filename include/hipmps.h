@@ -75,7 +75,8 @@ typedef struct {
   int device;               /* HIP device ordinal */
   size_t hbm_budget_bytes;  /* admission budget; 0 = 90% of the device's memory */
   int jacobi_max_sweeps;    /* 0 = default (40) */
-  double jacobi_tol;        /* 0 = default (relative off-diagonal 1e-13 fp64) */
+  double jacobi_tol;        /* 0 = default: relative off-diagonal 1e-13 (fp64 states) */
+  double jacobi_tol_mixed;  /* 0 = default: 1e-8 (mixed states, factors kept in complex64) */
 } hipmps_engine_opts;
 
 typedef struct {
@@ -185,6 +186,9 @@ typedef struct {
   double jacobi_flops; /* flops executed by the Jacobi block-rotation kernel */
   long long waves;      /* batched waves run */
   long long wave_tasks; /* 2-qubit gates applied in those waves */
+  long long jacobi_sweeps; /* sweeps summed over all SVDs */
+  long long svds_large;    /* SVDs with min(m, n) >= 256 */
+  long long sweeps_large;  /* sweeps summed over those */
 } hipmps_profile_t;
 int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset);
 int hipmps_engine_set_timing(hipmps_engine e, int enabled);

@@ -45,7 +45,8 @@ class HipMpsError(RuntimeError):
 
 class EngineOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("hbm_budget_bytes", ctypes.c_size_t),
-                ("jacobi_max_sweeps", ctypes.c_int), ("jacobi_tol", ctypes.c_double)]
+                ("jacobi_max_sweeps", ctypes.c_int), ("jacobi_tol", ctypes.c_double),
+                ("jacobi_tol_mixed", ctypes.c_double)]
 
 
 class TruncOpts(ctypes.Structure):
@@ -82,7 +83,9 @@ class Profile(ctypes.Structure):
                 ("ms_other", ctypes.c_double), ("ms_jacobi_rounds", ctypes.c_double),
                 ("jacobi_launches", ctypes.c_longlong), ("jacobi_pairs_rotated", ctypes.c_longlong),
                 ("jacobi_pairs_visited", ctypes.c_longlong), ("jacobi_flops", ctypes.c_double),
-                ("waves", ctypes.c_longlong), ("wave_tasks", ctypes.c_longlong)]
+                ("waves", ctypes.c_longlong), ("wave_tasks", ctypes.c_longlong),
+                ("jacobi_sweeps", ctypes.c_longlong), ("svds_large", ctypes.c_longlong),
+                ("sweeps_large", ctypes.c_longlong)]
 
 
 _lib = None

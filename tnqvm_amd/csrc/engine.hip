@@ -155,7 +155,8 @@ struct hipmps_engine_s {
   size_t budget = 0;
   size_t used = 0;  // bytes of site storage held by states
   int max_sweeps = 40;
-  double tol = 1e-13;
+  double tol = 1e-13;       // FP64 path: relative orthogonality of converged rows
+  double tol_mixed = 1e-8;  // mixed path: factors are stored in complex64
   int inner_sweeps = 1;
   DevBuf ws;     // per-wave arena
   DevBuf desc;   // task descriptors
@@ -357,9 +358,9 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     it.off_theta = off;
     off = align_up(off + (size_t)it.m * it.n * 16);
     it.off_y = off;
-    off = align_up(off + (size_t)it.r_pad * (it.c + it.r) * 16);
+    off = align_up(off + (size_t)it.r_pad * it.c * 16);
     it.off_vec = off;
-    off = align_up(off + (size_t)it.r_pad * (8 * 4 + 4));
+    off = align_up(off + (size_t)it.r_pad * (8 * 5 + 4));
   }
   RC_TRY(E->ws.ensure(off));
   char* ws = (char*)E->ws.p;
@@ -371,7 +372,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   double* d_w = (double*)((char*)E->small.p + align_up(3 * N * sizeof(int)));
   // descriptors
   size_t desc_bytes = align_up(N * sizeof(ThetaTask)) + align_up(N * sizeof(SplitTask)) +
-                      align_up(N * sizeof(JacTask));
+                      align_up(N * sizeof(JacTask)) + align_up(N * sizeof(GemmTask));
   RC_TRY(E->desc.ensure(desc_bytes));
   // host staging: descriptors | counts (2N ints) | active Jacobi list / results
   const size_t stage_bytes = desc_bytes + align_up(2 * N * sizeof(int)) + N * sizeof(JacTask) + 4 * N * sizeof(double) + 4096;
@@ -380,10 +381,12 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   ThetaTask* hth = (ThetaTask*)hd;
   SplitTask* hsp = (SplitTask*)(hd + align_up(N * sizeof(ThetaTask)));
   JacTask* hjt = (JacTask*)((char*)hsp + align_up(N * sizeof(SplitTask)));
+  GemmTask* hgm = (GemmTask*)((char*)hjt + align_up(N * sizeof(JacTask)));
   char* dd = (char*)E->desc.p;
   ThetaTask* dth = (ThetaTask*)dd;
   SplitTask* dsp = (SplitTask*)(dd + align_up(N * sizeof(ThetaTask)));
   JacTask* djt = (JacTask*)((char*)dsp + align_up(N * sizeof(SplitTask)));
+  GemmTask* dgm = (GemmTask*)((char*)djt + align_up(N * sizeof(JacTask)));
 
   // group boundaries
   int n64 = 0;
@@ -415,14 +418,15 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     S.n = it.n;
     S.r = it.r;
     S.c = it.c;
-    S.ldy = it.c + it.r;
+    S.ldy = it.c;
     S.transposed = it.transposed;
     char* v = ws + it.off_vec;
     S.rownorm = (double*)v;
     S.sig = (double*)(v + 8 * it.r_pad);
     S.scale_y = (double*)(v + 16 * it.r_pad);
     S.scale_w = (double*)(v + 24 * it.r_pad);
-    S.perm = (int*)(v + 32 * it.r_pad);
+    S.inv_sig = (double*)(v + 32 * it.r_pad);
+    S.perm = (int*)(v + 40 * it.r_pad);
     S.K = d_K + q;
     S.wstats = d_w + 4 * q;
     S.Ai = it.s->sites[it.i].p;
@@ -443,15 +447,56 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     J.r = it.r;
     J.r_pad = it.r_pad;
     J.c = it.c;
-    J.c_tot = it.c + it.r;
+    J.c_tot = it.c;
     J.nb = it.r_pad / JB;
     J.counter = d_counts + 2 * q;
     J.visited = d_counts + 2 * q + 1;
+    J.tol = it.fp32 ? E->tol_mixed : E->tol;
 
     max_r = std::max(max_r, it.r);
     max_rpad = std::max(max_rpad, it.r_pad);
-    max_c = std::max(max_c, it.c + it.r);
+    max_c = std::max(max_c, it.c);
     max_k = std::max(max_k, it.kub);
+  }
+  // GEMM recovering the second factor, one per item; tile_begin relative to its
+  // (precision, transposed) range
+  struct Range {
+    int b, e, tiles;
+  };
+  std::vector<Range> granges;
+  for (int q = 0; q < N;) {
+    int e = q;
+    while (e < N && items[e].fp32 == items[q].fp32 && items[e].transposed == items[q].transposed) ++e;
+    int tiles = 0;
+    for (int k = q; k < e; ++k) {
+      auto& it = items[k];
+      SplitTask& S = hsp[k];
+      GemmTask& G = hgm[k];
+      memset(&G, 0, sizeof(G));
+      if (!it.transposed) {
+        // Ai (m x K) = theta (m x n) . Ysel^H (n x K) . diag(1/sigma)
+        G.A = S.theta; G.lda = it.n;
+        G.B = S.Y; G.ldb = S.ldy; G.gatherB = S.perm;
+        G.C = S.Ai; G.ldc = -1;
+        G.M = it.m; G.N = it.kub; G.K = it.n;
+        G.Ndyn = S.K;
+        G.colscale = S.inv_sig;
+      } else {
+        // Aj (K x n) = diag(1/sigma) . Ysel (K x m) . theta (m x n)
+        G.A = S.Y; G.lda = S.ldy; G.gatherA = S.perm;
+        G.B = S.theta; G.ldb = it.n;
+        G.C = S.Aj; G.ldc = it.n;
+        G.M = it.kub; G.N = it.n; G.K = it.m;
+        G.Mdyn = S.K;
+        G.rowscale = S.inv_sig;
+      }
+      G.tiles_m = (G.M + 31) / 32;
+      G.tiles_n = (G.N + 31) / 32;
+      G.tile_begin = tiles;
+      tiles += G.tiles_m * G.tiles_n;
+    }
+    granges.push_back({q, e, tiles});
+    q = e;
   }
   HIP_TRY(hipMemcpyAsync(dd, hd, desc_bytes, hipMemcpyHostToDevice, st));
 
@@ -484,7 +529,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * N * sizeof(int), st));
     if (E->timing) HIP_TRY(hipEventRecord(E->ev[4], st));
     for (int round = 0; round < max_rounds; ++round)
-      HIP_TRY(launch_jacobi_round(djt, nact, pairs, round, E->tol, E->inner_sweeps, st));
+      HIP_TRY(launch_jacobi_round(djt, nact, pairs, round, E->inner_sweeps, st));
     if (E->timing) HIP_TRY(hipEventRecord(E->ev[5], st));
     E->prof.jacobi_launches += max_rounds;
     HIP_TRY(hipMemcpyAsync(hcounts, d_counts, 2 * N * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -500,8 +545,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       long long rot = hcounts[2 * q], vis = hcounts[2 * q + 1];
       E->prof.jacobi_pairs_rotated += rot;
       E->prof.jacobi_pairs_visited += vis;
-      const double c = items[q].c, ct = items[q].c + items[q].r;
-      E->prof.jacobi_flops += 8.0 * JP * JP * (c * (double)vis + ct * (double)rot);
+      const double c = items[q].c;
+      E->prof.jacobi_flops += 8.0 * JP * JP * c * (double)(vis + rot);
       if (rot == 0) sweeps_used[q] = sweep + 1;
       else next.push_back(q);
     }
@@ -523,6 +568,12 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     }
     if (n64 > 0) HIP_TRY(launch_gather(dsp, n64, mk64, max_len64, 0, st));
     if (N - n64 > 0) HIP_TRY(launch_gather(dsp + n64, N - n64, mk32, max_len32, 1, st));
+    for (auto& R : granges) {
+      const auto& it = items[R.b];
+      HIP_TRY(launch_gemm(dgm + R.b, R.e - R.b, R.tiles, 0, it.transposed ? 0 : 2, 0, 0, it.fp32 ? 1 : 0, st));
+    }
+    if (n64 > 0) HIP_TRY(launch_normalize(dsp, n64, mk64, 0, st));
+    if (N - n64 > 0) HIP_TRY(launch_normalize(dsp + n64, N - n64, mk32, 1, st));
   }
   // results
   int* hK = hcounts;  // reuse staging
@@ -542,6 +593,13 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   }
   E->prof.waves += 1;
   E->prof.wave_tasks += N;
+  for (int q = 0; q < N; ++q) {
+    E->prof.jacobi_sweeps += sweeps_used[q];
+    if (items[q].r >= 256) {
+      E->prof.svds_large += 1;
+      E->prof.sweeps_large += sweeps_used[q];
+    }
+  }
   for (int q = 0; q < N; ++q) {
     auto& it = items[q];
     hipmps_state_s* s = it.s;
@@ -1079,6 +1137,7 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
   e->budget = (opts && opts->hbm_budget_bytes) ? opts->hbm_budget_bytes : (size_t)(0.9 * (double)totalb);
   if (opts && opts->jacobi_max_sweeps > 0) e->max_sweeps = opts->jacobi_max_sweeps;
   if (opts && opts->jacobi_tol > 0) e->tol = opts->jacobi_tol;
+  if (opts && opts->jacobi_tol_mixed > 0) e->tol_mixed = opts->jacobi_tol_mixed;
   for (auto& ev : e->ev) (void)hipEventCreate(&ev);
   *out = e;
   return 0;

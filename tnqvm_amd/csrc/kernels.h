@@ -27,31 +27,36 @@ struct ThetaTask {
   double2 G[16];     // G[(a,b)*4 + (a',b')], a = bit of the left site
 };
 
-// One Jacobi problem on Y = [X | I_r] (r_pad x (c + r), row-major, ld = ldy): the rows
-// are orthogonalised with respect to their first c columns (the data X) and every
-// rotation is applied to all c + r columns, so the last r columns accumulate the
-// rotation product W^T (rows of the final Y: [sigma_k v_k^H | W[:,k]^T]).
+// One Jacobi problem: orthogonalise the rows of Y (r_pad x c_tot, row-major, ld = ldy)
+// with respect to their first c columns (the data X); every rotation is applied to all
+// c_tot columns (c_tot = c: the rotation product is not carried).
 struct JacTask {
   double2* Y;
   int ldy, r, r_pad, c;
-  int c_tot;         // c + r
+  int c_tot;
   int nb;            // r_pad / JB (even)
   int pair_begin;    // first block of this task in the grid
   int* counter;      // rotated block pairs in the current sweep
   int* visited;      // block pairs visited in the current sweep
+  double tol;        // relative off-diagonal below which a block pair counts as orthogonal
 };
 
 // Everything needed after convergence: singular values, truncation, factors.
+// Split after convergence.  Row k of the final Y is sigma_k xh_k^H (xh_k the k-th right
+// singular vector of X); the other side's singular vector is recovered as
+// normalize(X xh_k), whose norm is fixed to 1 so that noise in the direction of a
+// numerically-zero singular value cannot be amplified.
 struct SplitTask {
   const double2* theta;  // m x n (ld n)
-  const double2* Y;      // r_pad x (c + r) (ld ldy): final rows [sigma_k x_k | w_k]
+  const double2* Y;      // r_pad x c (ld ldy): final rows sigma_k xh_k^H
   int m, n, r, c, ldy;
   int transposed;        // 0: X = theta (m <= n); 1: X = theta^H (m > n)
   double* rownorm;       // r
   double* sig;           // r (sorted, descending)
   int* perm;             // r
-  double* scale_y;       // K: factor built from the data part of row k
-  double* scale_w;       // K: factor built from the rotation part of row k
+  double* scale_y;       // K: factor gathered from row k of Y (sigma^-1 (f sigma)^a)
+  double* scale_w;       // K: target norm (f sigma)^a of the recovered factor
+  double* inv_sig;       // K: 1 / sigma_k (0 for sigma_k = 0), GEMM operand scale
   int* K;                // out: new bond dimension
   double* wstats;        // out: [discarded, kept, total, renorm factor]
   void* Ai;              // out site i: cl x 2 x K
@@ -106,11 +111,14 @@ hipError_t launch_theta(const ThetaTask* d_tasks, int ntasks, int total_tiles, i
 hipError_t launch_jacobi_prep(const SplitTask* d_split, const JacTask* d_jac, int ntasks,
                               int max_rows, int max_cols, hipStream_t s);
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
-                               double tol, int inner_sweeps, hipStream_t s);
+                               int inner_sweeps, hipStream_t s);
 hipError_t launch_rownorm(const SplitTask* d_split, int ntasks, int max_r, hipStream_t s);
 hipError_t launch_select(const SplitTask* d_split, int ntasks, hipStream_t s);
 hipError_t launch_gather(const SplitTask* d_split, int ntasks, int max_k, int max_len, int fp32,
                          hipStream_t s);
+// rescale the GEMM-recovered factor: columns of Ai (not transposed) or rows of Aj
+// (transposed) to norm scale_w[k]
+hipError_t launch_normalize(const SplitTask* d_split, int ntasks, int max_k, int fp32, hipStream_t s);
 // element types: 0 = complex128, 1 = complex64
 hipError_t launch_gemm(const GemmTask* d_tasks, int ntasks, int total_tiles, int opa, int opb,
                        int ta, int tb, int tc, hipStream_t s);

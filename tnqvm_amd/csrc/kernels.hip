@@ -184,7 +184,7 @@ hipError_t launch_theta(const ThetaTask* d_tasks, int ntasks, int total_tiles, i
 }
 
 // ------------------------------------------------------------------ Jacobi: prepare Y
-// Y = [X | I_r] with X = theta (m <= n) or theta^H (m > n); rows r..r_pad-1 are zero.
+// Y = X with X = theta (m <= n) or theta^H (m > n); rows r..r_pad-1 are zero.
 __global__ __launch_bounds__(256) void k_jacobi_prep(const SplitTask* sp, const JacTask* jt) {
   __shared__ double2 tile[32][33];
   const SplitTask& S = sp[blockIdx.y];
@@ -195,24 +195,12 @@ __global__ __launch_bounds__(256) void k_jacobi_prep(const SplitTask* sp, const 
   if (bt >= tiles_c * tiles_r) return;
   const int i0 = (bt / tiles_c) * 32, c0 = (bt % tiles_c) * 32;
   const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
-  if (c0 >= J.c) {
-    // identity part (whole tile right of the data)
-    for (int y = ty; y < 32; y += 8) {
-      int i = i0 + y, c = c0 + tx;
-      if (c < J.c_tot) J.Y[(size_t)i * J.ldy + c] = make_double2((i < J.r && c - J.c == i) ? 1.0 : 0.0, 0.0);
-    }
-    return;
-  }
   if (!S.transposed) {
     for (int y = ty; y < 32; y += 8) {
       int i = i0 + y, c = c0 + tx;
       if (c < J.c_tot) {
         double2 v = make_double2(0, 0);
-        if (c < J.c) {
-          if (i < J.r) v = S.theta[(size_t)i * S.n + c];
-        } else if (i < J.r && c - J.c == i) {
-          v = make_double2(1.0, 0.0);
-        }
+        if (i < J.r && c < J.c) v = S.theta[(size_t)i * S.n + c];
         J.Y[(size_t)i * J.ldy + c] = v;
       }
     }
@@ -227,11 +215,7 @@ __global__ __launch_bounds__(256) void k_jacobi_prep(const SplitTask* sp, const 
     __syncthreads();
     for (int y = ty; y < 32; y += 8) {
       int i = i0 + y, c = c0 + tx;
-      if (c < J.c_tot) {
-        double2 v = tile[tx][y];
-        if (c >= J.c) v = make_double2((i < J.r && c - J.c == i) ? 1.0 : 0.0, 0.0);
-        J.Y[(size_t)i * J.ldy + c] = v;
-      }
+      if (c < J.c_tot) J.Y[(size_t)i * J.ldy + c] = tile[tx][y];
     }
   }
 }
@@ -288,7 +272,7 @@ __device__ __forceinline__ mfma_d4 mfma64(double a, double b, mfma_d4 c) {
 // Column chunk of the 2*JB rows staged in LDS; 32 columns per chunk.
 constexpr int JC = 32;
 
-__global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int ntasks, int round, double tol,
+__global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int ntasks, int round,
                                                       int inner_sweeps) {
   __shared__ double2 sY[JP][JC + 1];
   __shared__ double2 sG[JP][JP + 1];
@@ -306,6 +290,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
   const int ba = circle_slot(pi, round, nb), bb = circle_slot(nb - 1 - pi, round, nb);
   const int C = T.c, CT = T.c_tot;
   const size_t ld = T.ldy;
+  const double tol = T.tol;
   double2* Y = T.Y;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -457,11 +442,10 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
   if (tid == 0) atomicAdd(T.counter, 1);
 }
 
-hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round, double tol,
+hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
                                int inner_sweeps, hipStream_t s) {
   if (total_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round, tol,
-                     inner_sweeps);
+  hipLaunchKernelGGL(k_jacobi_round, dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round, inner_sweeps);
   return hipGetLastError();
 }
 
@@ -553,8 +537,8 @@ __global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
   double f = 1.0;
   if (S.renorm && kept > 0.0) f = sqrt((kept + disc) / kept);
   // scales: left factor U (s)^aL, right factor (s)^aR V^H with s = f * sigma.
-  // Row k of Y is [sigma_k x_k | w_k]: the data part gives the factor on the x side
-  // (times sigma^-1), the rotation part the other factor (a unit vector).
+  // Row k of Y (sigma_k xh_k^H) gives the factor on the X-row side (times sigma^-1);
+  // the other factor is a unit vector recovered by a GEMM and scaled to (s)^a.
   const double aL = S.absorb == 0 ? 0.5 : 0.0, aR = 1.0 - aL;
   const double a_y = S.transposed ? aL : aR;
   const double a_w = S.transposed ? aR : aL;
@@ -563,16 +547,16 @@ __global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
     S.perm[k] = idx[k];
     if (k < K) {
       double sg = key[k];
-      double gy = 0.0, gw = 0.0;
+      double gy = 0.0, gw = 0.0, is = 0.0;
       if (sg > 1e-150) {
         double sf = sg * f;
         gy = pow(sf, a_y) / sg;
         gw = pow(sf, a_w);
-      } else {
-        gw = a_w == 0.0 ? 1.0 : 0.0;
+        is = 1.0 / sg;
       }
       S.scale_y[k] = gy;
       S.scale_w[k] = gw;
+      S.inv_sig[k] = is;
     }
   }
   if (tid == 0) {
@@ -589,11 +573,9 @@ hipError_t launch_select(const SplitTask* d_split, int ntasks, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ factors from Y rows
-// blockIdx.z = 0: row gather     dst[k][j] = Y[perm k][off + j] * scale_k      (K x len)
-// blockIdx.z = 1: conj-transpose dst[i][k] = conj(Y[perm k][off + i]) * scale_k (len x K)
-// not transposed (X = theta):  Aj <- rows of the data part (scale_y),  Ai <- conj W (scale_w)
-// transposed (X = theta^H):    Aj <- rows of the W part (scale_w),     Ai <- conj data (scale_y)
+// ------------------------------------------------------------------ factor from Y rows
+// not transposed (X = theta):  Aj[k][j] = Y[perm k][j] * scale_y[k]         (K x n)
+// transposed (X = theta^H):    Ai[i][k] = conj(Y[perm k][i]) * scale_y[k]   (m x K)
 template <typename T>
 __global__ __launch_bounds__(256) void k_gather(const SplitTask* sp) {
   __shared__ double2 tile[32][33];
@@ -601,10 +583,8 @@ __global__ __launch_bounds__(256) void k_gather(const SplitTask* sp) {
   const int K = *S.K;
   const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
   const int bt = blockIdx.x;
-  if (blockIdx.z == 0) {
-    const int len = S.n;                       // Aj is K x n
-    const int off = S.transposed ? S.c : 0;
-    const double* sc = S.transposed ? S.scale_w : S.scale_y;
+  if (!S.transposed) {
+    const int len = S.n;
     const int tiles_c = (len + 31) / 32;
     const int k0 = (bt / tiles_c) * 32, c0 = (bt % tiles_c) * 32;
     if (k0 >= K) return;
@@ -612,14 +592,12 @@ __global__ __launch_bounds__(256) void k_gather(const SplitTask* sp) {
     for (int y = ty; y < 32; y += 8) {
       int k = k0 + y, c = c0 + tx;
       if (k < K && c < len) {
-        double2 v = S.Y[(size_t)S.perm[k] * S.ldy + off + c];
-        st(out, (size_t)k * len + c, cscale(v, sc[k]));
+        double2 v = S.Y[(size_t)S.perm[k] * S.ldy + c];
+        st(out, (size_t)k * len + c, cscale(v, S.scale_y[k]));
       }
     }
   } else {
-    const int len = S.m;                       // Ai is m x K
-    const int off = S.transposed ? 0 : S.c;
-    const double* sc = S.transposed ? S.scale_y : S.scale_w;
+    const int len = S.m;
     const int tiles_i = (len + 31) / 32;
     const int k0 = (bt / tiles_i) * 32, i0 = (bt % tiles_i) * 32;
     if (k0 >= K) return;
@@ -627,7 +605,7 @@ __global__ __launch_bounds__(256) void k_gather(const SplitTask* sp) {
     for (int y = ty; y < 32; y += 8) {
       int k = k0 + y, i = i0 + tx;
       double2 v = make_double2(0, 0);
-      if (k < K && i < len) v = cscale(cconj(S.Y[(size_t)S.perm[k] * S.ldy + off + i]), sc[k]);
+      if (k < K && i < len) v = cscale(cconj(S.Y[(size_t)S.perm[k] * S.ldy + i]), S.scale_y[k]);
       tile[y][tx] = v;
     }
     __syncthreads();
@@ -641,11 +619,70 @@ __global__ __launch_bounds__(256) void k_gather(const SplitTask* sp) {
 hipError_t launch_gather(const SplitTask* d_split, int ntasks, int max_k, int max_len, int fp32,
                          hipStream_t s) {
   int tiles = ((max_k + 31) / 32) * ((max_len + 31) / 32);
-  dim3 grid(tiles, ntasks, 2);
   if (fp32)
-    hipLaunchKernelGGL(k_gather<float2>, grid, dim3(256), 0, s, d_split);
+    hipLaunchKernelGGL(k_gather<float2>, dim3(tiles, ntasks), dim3(256), 0, s, d_split);
   else
-    hipLaunchKernelGGL(k_gather<double2>, grid, dim3(256), 0, s, d_split);
+    hipLaunchKernelGGL(k_gather<double2>, dim3(tiles, ntasks), dim3(256), 0, s, d_split);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ normalise recovered factor
+// not transposed: columns k of Ai (m x K) scaled to norm scale_w[k];
+// transposed:     rows k of Aj (K x n) scaled to norm scale_w[k].
+// A vector that came out exactly zero stays zero.
+template <typename T>
+__global__ __launch_bounds__(256) void k_normalize(const SplitTask* sp) {
+  __shared__ double part[8][33];
+  __shared__ double fac[32];
+  const SplitTask& S = sp[blockIdx.y];
+  const int K = *S.K;
+  const int k0 = blockIdx.x * 32;
+  if (k0 >= K) return;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+  if (!S.transposed) {
+    T* A = (T*)S.Ai;
+    const int k = k0 + tx;
+    double a = 0.0;
+    if (k < K)
+      for (int i = ty; i < S.m; i += 8) a += cabs2(ld(A, (size_t)i * K + k));
+    part[ty][tx] = a;
+    __syncthreads();
+    if (ty == 0) {
+      double t = 0.0;
+      for (int q = 0; q < 8; ++q) t += part[q][tx];
+      fac[tx] = (k < K && t > 0.0) ? S.scale_w[k] / sqrt(t) : 0.0;
+    }
+    __syncthreads();
+    if (k < K)
+      for (int i = ty; i < S.m; i += 8) {
+        size_t o = (size_t)i * K + k;
+        st(A, o, cscale(ld(A, o), fac[tx]));
+      }
+  } else {
+    T* A = (T*)S.Aj;
+    // 8 rows per block pass, 32 threads per row
+    for (int kr = ty; kr < 32; kr += 8) {
+      const int k = k0 + kr;
+      double a = 0.0;
+      if (k < K)
+        for (int j = tx; j < S.n; j += 32) a += cabs2(ld(A, (size_t)k * S.n + j));
+      for (int off = 16; off > 0; off >>= 1) a += __shfl_xor(a, off, 32);
+      const double fk = (k < K && a > 0.0) ? S.scale_w[k] / sqrt(a) : 0.0;
+      if (k < K)
+        for (int j = tx; j < S.n; j += 32) {
+          size_t o = (size_t)k * S.n + j;
+          st(A, o, cscale(ld(A, o), fk));
+        }
+    }
+  }
+}
+
+hipError_t launch_normalize(const SplitTask* d_split, int ntasks, int max_k, int fp32, hipStream_t s) {
+  dim3 grid((max_k + 31) / 32, ntasks);
+  if (fp32)
+    hipLaunchKernelGGL(k_normalize<float2>, grid, dim3(256), 0, s, d_split);
+  else
+    hipLaunchKernelGGL(k_normalize<double2>, grid, dim3(256), 0, s, d_split);
   return hipGetLastError();
 }
 

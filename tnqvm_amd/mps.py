@@ -66,8 +66,8 @@ def trunc_opts(cutoff=DBL_MIN, max_bond=0, precision="fp64", renormalize=False, 
 
 class Engine:
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, jacobi_max_sweeps: int = 0,
-                 jacobi_tol: float = 0.0):
-        o = C.EngineOpts(device, hbm_budget_bytes, jacobi_max_sweeps, jacobi_tol)
+                 jacobi_tol: float = 0.0, jacobi_tol_mixed: float = 0.0):
+        o = C.EngineOpts(device, hbm_budget_bytes, jacobi_max_sweeps, jacobi_tol, jacobi_tol_mixed)
         h = ctypes.c_void_p()
         check(lib().hipmps_engine_create(ctypes.byref(o), ctypes.byref(h)))
         self.h = h
