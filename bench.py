@@ -100,6 +100,31 @@ def serving_latency(eng, n_circuits, depth, rate_per_step, precision, seed):
     return np.array(lat), wall
 
 
+def gate_at_a_time(eng, circuit, warmup, steps, max_applies=40):
+    """Plain FP64, one 2-qubit gate per wave (the reference visitor's gate-by-gate model):
+    the circuit is brought to layer `warmup` normally, then 2-qubit gates of the timed layers
+    are applied and flushed one at a time; a bounded sample of `max_applies` gates is timed."""
+    s = mps.MpsState(eng, N_QUBITS, cutoff=CUTOFF, max_bond=CHI, precision="fp64")
+    per_layer = C.brickwork_layers(circuit, N_QUBITS)
+    for l in range(warmup):
+        s.run(per_layer[l])
+    s.flush()
+    done, t0 = 0, time.perf_counter()
+    for l in range(warmup, warmup + steps):
+        for g in per_layer[l]:
+            s.run([g])
+            if len(g["qubits"]) == 2:
+                s.flush()
+                done += 1
+                if done >= max_applies:
+                    break
+        if done >= max_applies:
+            break
+    el = time.perf_counter() - t0
+    s.close()
+    return done, el
+
+
 def cpu_baseline(seconds=12.0):
     """FP64 oracle (CPU restatement of the exatn-mps algorithm) at chi = 256: Haar gates on the
     two middle bonds of an 18-site chain whose bonds are min(2^k, 2^(18-k), 256)."""
@@ -125,18 +150,40 @@ def cpu_baseline(seconds=12.0):
                       f"18-site FP64 MPS, {el:.1f} s, oracle/mps_oracle.cpp with {threads} OpenMP threads"}
 
 
+def calibrate_copy(nbytes: int, reps: int = 4):
+    """Plain device-to-device copies of a known size, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings (run under the profiler: bench.py --calibrate-copy BYTES)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=28, help="timed brickwork layers")
     ap.add_argument("--warmup", type=int, default=12, help="untimed layers (bond growth to chi=256)")
-    ap.add_argument("--batch", type=int, default=4, help="circuits advanced together")
+    ap.add_argument("--batch", type=int, default=8, help="circuits advanced together")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-circuits", type=int, default=12)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run plain device copies of BYTES (profiler counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent circuits shard "
               "across GPUs with no communication, so the per-GPU figure at N=1 is the measurement.",
@@ -156,8 +203,16 @@ def main():
     # roofline of the dominant kernel (k_jacobi_round): executed MFMA flops / event-timed launch spans
     jac_tflops = prof["jacobi_flops"] / max(prof["ms_jacobi_rounds"], 1e-9) / 1e9
     avg_launch_us = 1e3 * prof["ms_jacobi_rounds"] / max(prof["jacobi_launches"], 1)
+    # HBM bytes per launch: per-block-pair FETCH/WRITE from the committed rocprofv3 counter runs
+    # (profiles/k_jacobi_round_traffic.json) times the block pairs this run launched per dispatch
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_jacobi_round_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        pairs_per_launch = prof["jacobi_pairs_visited"] / max(prof["jacobi_launches"], 1)
+        traffic = round((tj["fetch_bytes_per_pair"] + tj["write_bytes_per_pair"]) * pairs_per_launch)
     roofline = {"bound": "mfma", "achieved": round(jac_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(jac_tflops / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(jac_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_jacobi_round", "avg_launch_us": round(avg_launch_us, 2),
                 "launches": prof["jacobi_launches"],
                 "flops_per_launch": prof["jacobi_flops"] / max(prof["jacobi_launches"], 1)}
@@ -178,11 +233,16 @@ def main():
         "roofline": roofline,
     }
     if not args.no_plain:
-        log("plain fp64, one circuit at a time")
+        log("plain fp64, one circuit at a time (bonds of a layer batched)")
         pa, pt, _, pprof, _ = run_layers(eng, circuits[:1], "fp64", args.warmup, args.steps)
         out["plain_fp64_value"] = round(pa / pt, 2)
         out["speedup_vs_plain"] = round(value / (pa / pt), 3)
         log(f"  {pa} applies in {pt:.3f} s -> {pa / pt:.1f} applies/s")
+        log("plain fp64, one circuit, one 2-qubit gate at a time")
+        ga, gt = gate_at_a_time(eng, circuits[0], args.warmup, args.steps)
+        out["plain_fp64_gate_at_a_time_value"] = round(ga / gt, 2)
+        out["speedup_vs_gate_at_a_time"] = round(value / (ga / gt), 3)
+        log(f"  {ga} applies in {gt:.3f} s -> {ga / gt:.1f} applies/s")
     if not args.no_serving:
         log(f"serving: {args.serve_circuits} circuits (depth 20), Poisson arrivals")
         lat, wall = serving_latency(eng, args.serve_circuits, 20, 0.5, args.precision, 0x5E7)

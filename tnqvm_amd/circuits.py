@@ -56,6 +56,18 @@ def haar_brickwork(n: int = 40, depth: int = 40, seed: int = 0x7E5A0002):
     return gates
 
 
+def brickwork_layers(gates, n: int):
+    """Split a brickwork circuit (haar_brickwork / u3_cnot_line) into its layers: layer l
+    holds n one-qubit gates followed by the 2-qubit gates on pairs starting at l % 2."""
+    out, pos, layer = [], 0, 0
+    while pos < len(gates):
+        size = n + len(range(layer % 2, n - 1, 2))
+        out.append(gates[pos:pos + size])
+        pos += size
+        layer += 1
+    return out
+
+
 def random_bitstrings(n: int, k: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return rng.integers(0, 2, size=(k, n), dtype=np.int8)
