@@ -10,6 +10,7 @@ Workloads (all seeded; the seed is recorded by the callers):
   * ``haar_brickwork`` Haar-random 1q/2q brickwork (40 qubits, depth 40 at full size)
   * ``qaoa_maxcut``    QAOA on a random 3-regular graph routed by a swap network
   * ``grid_fsim``      6x6 grid fSim circuits, snake-mapped onto a line
+  * ``mixed_batch``    many brickwork circuits of uneven width, depth and chi_max
 """
 from __future__ import annotations
 
@@ -68,6 +69,37 @@ def brickwork_layers(gates, n: int):
     return out
 
 
+def route_nearest_neighbour(gates):
+    """Replace every 2-qubit gate on qubits at distance > 1 by swaps that move the two
+    qubits towards each other from both ends, the gate on the met pair, and the inverse
+    swaps (NearestNeighborTransform.hpp:77-121, the same scheme as the hip-mps visitor)."""
+    out = []
+    for g in gates:
+        q = g["qubits"]
+        if len(q) != 2 or abs(q[0] - q[1]) <= 1:
+            out.append(g)
+            continue
+        lower0, upper0 = min(q), max(q)
+        lo, up = lower0, upper0
+        while True:
+            out.append({"name": "Swap", "qubits": (lo, lo + 1), "params": ()})
+            lo += 1
+            if up - lo <= 1:
+                break
+            out.append({"name": "Swap", "qubits": (up, up - 1), "params": ()})
+            up -= 1
+            if up - lo <= 1:
+                break
+        moved = dict(g)
+        moved["qubits"] = (lo, up) if q[0] < q[1] else (up, lo)
+        out.append(moved)
+        for i in range(lo, lower0, -1):
+            out.append({"name": "Swap", "qubits": (i, i - 1), "params": ()})
+        for i in range(up, upper0):
+            out.append({"name": "Swap", "qubits": (i, i + 1), "params": ()})
+    return out
+
+
 def random_bitstrings(n: int, k: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return rng.integers(0, 2, size=(k, n), dtype=np.int8)
@@ -91,4 +123,132 @@ def layers_2q(gates):
         busy |= qs
     if cur:
         out.append(cur)
+    return out
+
+
+# ------------------------------------------------------------------ QAOA MaxCut
+def random_regular_graph(n: int, d: int, rng: np.random.Generator, max_tries: int = 1000):
+    """Seeded d-regular simple graph by the pairing (configuration) model with rejection."""
+    if (n * d) % 2:
+        raise ValueError("n * d must be even")
+    for _ in range(max_tries):
+        stubs = np.repeat(np.arange(n), d)
+        rng.shuffle(stubs)
+        pairs = stubs.reshape(-1, 2)
+        if np.any(pairs[:, 0] == pairs[:, 1]):
+            continue
+        edges = {tuple(sorted(map(int, p))) for p in pairs}
+        if len(edges) == len(pairs):
+            return sorted(edges)
+    raise RuntimeError("no simple regular graph found")
+
+
+def zz_phase(gamma: float) -> np.ndarray:
+    """exp(-i gamma Z x Z) = CNOT . (I x Rz(2 gamma)) . CNOT."""
+    return np.diag(np.exp(-1j * gamma * np.array([1, -1, -1, 1]))).astype(np.complex128)
+
+
+SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=np.complex128)
+
+
+def qaoa_maxcut(n: int = 60, p: int = 4, seed: int = 0x7E5A0003):
+    """QAOA MaxCut on a seeded random 3-regular graph, gamma, beta ~ U[0, pi].
+    Each cost layer runs an odd-even transposition swap network (n layers of alternating
+    neighbour swaps): every logical pair becomes adjacent exactly once, and an edge's ZZ
+    phase is fused with the swap that passes it (one 4x4 gate).  Mixer: Rx(2 beta).
+    Returns (gates, edges, final_position) with final_position[logical] = physical site."""
+    rng = np.random.default_rng(seed)
+    edges = random_regular_graph(n, 3, rng)
+    eset = set(edges)
+    gammas = rng.uniform(0.0, np.pi, size=p)
+    betas = rng.uniform(0.0, np.pi, size=p)
+    gates = [{"name": "H", "qubits": (q,), "params": ()} for q in range(n)]
+    phys = list(range(n))  # phys[site] = logical qubit on that site
+    for layer in range(p):
+        zz = zz_phase(gammas[layer])
+        fused = SWAP @ zz
+        for t in range(n):
+            for s0 in range(t % 2, n - 1, 2):
+                a, b = phys[s0], phys[s0 + 1]
+                m = fused if tuple(sorted((a, b))) in eset else SWAP
+                gates.append({"name": "unitary", "qubits": (s0, s0 + 1), "matrix": m})
+                phys[s0], phys[s0 + 1] = b, a
+        for q in range(n):
+            gates.append({"name": "Rx", "qubits": (q,), "params": (float(2.0 * betas[layer]),)})
+    pos = [0] * n
+    for site, logical in enumerate(phys):
+        pos[logical] = site
+    return gates, edges, pos, (gammas, betas)
+
+
+def qaoa_logical(n: int, edges, gammas, betas):
+    """The same QAOA circuit on logical qubits without routing (for dense checks)."""
+    gates = [{"name": "H", "qubits": (q,), "params": ()} for q in range(n)]
+    for g, b in zip(gammas, betas):
+        for (i, j) in edges:
+            gates.append({"name": "unitary", "qubits": (i, j), "matrix": zz_phase(g)})
+        for q in range(n):
+            gates.append({"name": "Rx", "qubits": (q,), "params": (float(2.0 * b),)})
+    return gates
+
+
+# ------------------------------------------------------------------ 2D grid fSim
+def snake_site(r: int, c: int, cols: int) -> int:
+    return r * cols + (c if r % 2 == 0 else cols - 1 - c)
+
+
+def grid_couplers(rows: int, cols: int):
+    """Coupler sets A, B (horizontal, even / odd column) and C, D (vertical, even / odd row)."""
+    A = [((r, c), (r, c + 1)) for r in range(rows) for c in range(0, cols - 1, 2)]
+    B = [((r, c), (r, c + 1)) for r in range(rows) for c in range(1, cols - 1, 2)]
+    Cv = [((r, c), (r + 1, c)) for r in range(0, rows - 1, 2) for c in range(cols)]
+    D = [((r, c), (r + 1, c)) for r in range(1, rows - 1, 2) for c in range(cols)]
+    return {"A": A, "B": B, "C": Cv, "D": D}
+
+
+def grid_fsim(rows: int = 6, cols: int = 6, cycles: int = 8, seed: int = 0x7E5A0005,
+              theta: float = np.pi / 2, phi: float = np.pi / 6):
+    """Random grid circuit: each cycle applies sqrt(X) = Rx(pi/2), sqrt(Y) = Ry(pi/2) or
+    sqrt(W) = Rz(-pi/4) Rx(pi/2) Rz(pi/4) (never the same on a qubit twice in a row) to every
+    qubit, then fSim(theta, phi) on one coupler pattern (ABCDCDAB order).  Qubits are placed
+    on the line by the snake map site = cols*r + (c if r even else cols-1-c); vertical
+    couplers become non-neighbouring 2-qubit gates (route_nearest_neighbour makes the
+    circuit nearest-neighbour)."""
+    rng = np.random.default_rng(seed)
+    pats = grid_couplers(rows, cols)
+    order = "ABCDCDAB"
+    n = rows * cols
+    last = [-1] * n
+    gates = []
+    for cyc in range(cycles):
+        for q in range(n):
+            k = int(rng.integers(0, 3))
+            while k == last[q]:
+                k = int(rng.integers(0, 3))
+            last[q] = k
+            if k == 0:
+                gates.append({"name": "Rx", "qubits": (q,), "params": (np.pi / 2,)})
+            elif k == 1:
+                gates.append({"name": "Ry", "qubits": (q,), "params": (np.pi / 2,)})
+            else:
+                gates.append({"name": "Rz", "qubits": (q,), "params": (-np.pi / 4,)})
+                gates.append({"name": "Rx", "qubits": (q,), "params": (np.pi / 2,)})
+                gates.append({"name": "Rz", "qubits": (q,), "params": (np.pi / 4,)})
+        for (a, b) in pats[order[cyc % len(order)]]:
+            sa, sb = snake_site(*a, cols), snake_site(*b, cols)
+            gates.append({"name": "fSim", "qubits": (sa, sb), "params": (float(theta), float(phi))})
+    return gates
+
+
+# ------------------------------------------------------------------ batch of uneven circuits
+def mixed_batch(count: int = 256, seed: int = 0x7E5A0004, n_range=(16, 48), depth_range=(10, 60),
+                chis=(32, 64, 128, 256)):
+    """Brickwork circuits with n ~ U{n_range}, depth ~ U{depth_range}, chi_max from `chis`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        d = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        chi = int(rng.choice(chis))
+        out.append({"n": n, "depth": d, "chi": chi, "gates": haar_brickwork(n, d, seed=seed * 1000 + k)})
     return out
