@@ -228,36 +228,51 @@ hipError_t launch_jacobi_prep(const SplitTask* d_split, const JacTask* d_jac, in
 }
 
 // ------------------------------------------------------------------ Jacobi: one round
-// One block per block pair (rows of two JB-row blocks).  Round-robin ("circle")
+// One workgroup per block pair (the rows of two JB-row blocks).  Round-robin ("circle")
 // ordering: slot 0 holds block 0, slot s>0 holds block 1 + (s-1+round) mod (nb-1); pair
-// i couples slots i and nb-1-i, so every unordered block pair meets once per sweep.
+// i couples slots i and nb-1-i, so every unordered block pair meets once per sweep.  The
+// same ordering, with 2*JB single rows in place of blocks, drives the inner sweep.
 //
 // For the 2*JB rows:  Gram  Gam[i][j] = sum_c conj(Y[i][c]) Y[j][c]
-//                     Gam = W Lambda W^H  by cyclic two-sided Jacobi in LDS (FP64)
+//                     Gam ~ W Lambda W^H  by one cyclic two-sided Jacobi sweep (FP64)
 //                     Y' = W^T Y          (rows become mutually orthogonal)
-// Rotation for a Hermitian pair (alpha, beta, gamma = |gamma| e^{i phi}), Golub & Van
-// Loan Alg. 8.4.1 with a phase: zeta = (beta-alpha)/(2|gamma|),
-// t = sign(zeta)/(|zeta|+sqrt(1+zeta^2)), c = 1/sqrt(1+t^2), s = c t,
-// J = [[c, s e^{i phi}], [-s e^{-i phi}, c]].
 __device__ __forceinline__ int circle_slot(int s, int round, int nb) {
   return s == 0 ? 0 : 1 + (s - 1 + round) % (nb - 1);
 }
 
-__device__ __forceinline__ void jacobi_rotation(double alpha, double beta, double2 gamma, double tol,
-                                                double& c, double2& se) {
-  double g = sqrt(cabs2(gamma));
-  if (g == 0.0 || !(alpha > 0.0) || !(beta > 0.0) || g <= tol * sqrt(alpha * beta)) {
+// 1/sqrt(a) and 1/d: the hardware approximations refined by one Newton step, which
+// brings them to the last bit or two of FP64 (all that a rotation needs: only
+// c^2 + |s|^2 = 1 has to hold to working precision, not the angle itself).
+__device__ __forceinline__ double nr_rsq(double a) {
+  const double y = __builtin_amdgcn_rsq(a);
+  const double hy = 0.5 * a * y;
+  return fma(y, fma(-hy, y, 0.5), y);
+}
+__device__ __forceinline__ double nr_rcp(double d) {
+  const double x = __builtin_amdgcn_rcp(d);
+  return fma(x, fma(-d, x, 1.0), x);
+}
+
+// Rotation that annihilates gamma in the Hermitian 2x2 [[alpha, gamma], [conj(gamma), beta]]
+// (Golub & Van Loan Alg. 8.4.1 with a phase): h = beta - alpha,
+// tau = 2 sign(h) gamma / (|h| + sqrt(h^2 + 4|gamma|^2))  (|tau| = t, the smaller root),
+// c = 1/sqrt(1 + |tau|^2), se = c tau;  J = [[c, se], [-conj(se), c]].
+// Returns the identity when |gamma| <= tol sqrt(alpha beta) or a diagonal entry is 0.
+__device__ __forceinline__ void jacobi_rotation(double alpha, double beta, double2 gamma, double tol, double& c,
+                                                double2& se) {
+  const double g2 = cabs2(gamma);
+  if (!(alpha > 0.0) || !(beta > 0.0) || !(g2 > tol * tol * alpha * beta)) {
     c = 1.0;
     se = make_double2(0.0, 0.0);
     return;
   }
-  double zeta = (beta - alpha) / (2.0 * g);
-  double t;
-  if (fabs(zeta) > 1e150) t = 0.5 / zeta;
-  else t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  c = 1.0 / sqrt(1.0 + t * t);
-  double s = c * t;
-  se = make_double2(s * gamma.x / g, s * gamma.y / g);
+  const double h = beta - alpha;
+  const double a = fma(h, h, 4.0 * g2);
+  const double d = fabs(h) + a * nr_rsq(a);
+  const double f = (h >= 0.0 ? 2.0 : -2.0) * nr_rcp(d);
+  const double2 tau = make_double2(f * gamma.x, f * gamma.y);
+  c = nr_rsq(1.0 + cabs2(tau));
+  se = make_double2(c * tau.x, c * tau.y);
 }
 
 // FP64 MFMA v_mfma_f64_16x16x4f64 operand layout on gfx950 (measured):
@@ -269,17 +284,32 @@ __device__ __forceinline__ mfma_d4 mfma64(double a, double b, mfma_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Column chunk of the 2*JB rows staged in LDS; 32 columns per chunk.
-constexpr int JC = 32;
-
-__global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int ntasks, int round,
-                                                      int inner_sweeps) {
-  __shared__ double2 sY[JP][JC + 1];
+// Design notes (gfx950):
+//  * Gram and apply stream their MFMA operands straight from memory into registers; no
+//    LDS staging and no barriers in either loop.  The Gram is Hermitian, so only tiles
+//    (0,0), (0,1), (1,1) are formed; each wave covers every fourth 4-column unit and the
+//    four partial Grams are summed once in LDS.
+//  * In the apply each wave owns whole 32-row x 16-column tiles, so the in-place update
+//    never overwrites a column another wave still reads.
+//  * Inner sweep: thread (k, l) owns the 2x2 block of rotation pairs k and l, updates it
+//    two-sided in one pass (B' = Jk^H B Jl, G double-buffered in LDS), updates two W
+//    entries pairs, and -- when the next step's pair (p', q') has p' in pair k and q' in
+//    pair l -- also forms that pair's rotation from values it already holds.  One barrier
+//    per step; every LDS read of a step is issued before any arithmetic.
+//    In the circle ordering the pair owner is static: next pair 0 <- (0,1), next pair j
+//    <- (j+1, j-1) for 1 <= j <= 14, next pair 15 <- (15,14); and each row index moves
+//    to x+1 (31 -> 1) from one step to the next, except row 0.
+// BLK is the block size JB (a template parameter: the kernel's register allocation on
+// gfx950 comes out at 150 VGPRs, three waves per SIMD, in this form).
+template <int BLK>
+__global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, int ntasks, int round,
+                                                         int inner_sweeps) {
+  static_assert(BLK == 16 && JB == BLK && JP == 2 * BLK, "256 threads map onto 16 x 16 rotation-pair blocks");
   __shared__ double2 sG[JP][JP + 1];
+  __shared__ double2 sG2[JP][JP + 1];
   __shared__ double2 sW[JP][JP + 1];
-  __shared__ double s_c[JP / 2];
-  __shared__ double2 s_se[JP / 2];
-  __shared__ int s_p[JP / 2], s_q[JP / 2];
+  __shared__ double s_c[2][JP / 2 + 1];  // rotations of the current / next step;
+  __shared__ double2 s_se[2][JP / 2 + 1];  // entry JP/2 takes the writes of non-owners
   __shared__ double s_red[256];
 
   const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const JacTask& x) { return x.pair_begin; });
@@ -295,57 +325,81 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, k4 = lane >> 4;
-  auto grow = [&](int j) { return j < JB ? ba * JB + j : bb * JB + (j - JB); };
+  const double2* ra = Y + (size_t)(ba * JB + l16) * ld;  // pair row l16 (block a)
+  const double2* rb = Y + (size_t)(bb * JB + l16) * ld;  // pair row 16 + l16 (block b)
 
-  // chunk loader: 32 rows x 32 columns, 4 elements per thread, coalesced along columns
-  double2 pre[4];
-  auto load_chunk = [&](int c0, int climit) {
+  // ---- Gram tiles (0,0), (0,1), (1,1) over this wave's 4-column units u = wave, wave + 4, ... ----
+  mfma_d4 a00r = {0, 0, 0, 0}, a00i = {0, 0, 0, 0}, a01r = {0, 0, 0, 0}, a01i = {0, 0, 0, 0};
+  mfma_d4 a11r = {0, 0, 0, 0}, a11i = {0, 0, 0, 0};
+  {
+    const int nu = (C + 3) >> 2;
+    constexpr int PD = 2;
+    double2 x0[PD], x1[PD];
+    auto ld_unit = [&](int u, double2& v0, double2& v1) {
+      const int col = 4 * u + k4;
+      if (u < nu && col < C) {
+        v0 = ra[col];
+        v1 = rb[col];
+      } else {
+        v0 = make_double2(0, 0);
+        v1 = make_double2(0, 0);
+      }
+    };
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int e = tid + 256 * q;
-      int row = e >> 5, col = e & 31;
-      pre[q] = (c0 + col < climit) ? Y[(size_t)grow(row) * ld + c0 + col] : make_double2(0, 0);
+    for (int d = 0; d < PD; ++d) ld_unit(wave + 4 * d, x0[d], x1[d]);
+    for (int u0 = wave; u0 < nu; u0 += 4 * PD) {
+#pragma unroll
+      for (int d = 0; d < PD; ++d) {
+        const double2 v0 = x0[d], v1 = x1[d];
+        ld_unit(u0 + 4 * (d + PD), x0[d], x1[d]);
+        a00r = mfma64(v0.x, v0.x, a00r);
+        a01r = mfma64(v0.x, v1.x, a01r);
+        a11r = mfma64(v1.x, v1.x, a11r);
+        a00i = mfma64(v0.x, v0.y, a00i);
+        a01i = mfma64(v0.x, v1.y, a01i);
+        a11i = mfma64(v1.x, v1.y, a11i);
+        a00r = mfma64(v0.y, v0.y, a00r);
+        a01r = mfma64(v0.y, v1.y, a01r);
+        a11r = mfma64(v1.y, v1.y, a11r);
+        a00i = mfma64(-v0.y, v0.x, a00i);
+        a01i = mfma64(-v0.y, v1.x, a01i);
+        a11i = mfma64(-v1.y, v1.x, a11i);
+      }
     }
-  };
-  auto store_chunk = [&]() {
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int e = tid + 256 * q;
-      sY[e >> 5][e & 31] = pre[q];
-    }
-  };
-
-  // ---- Gram  G[i][j] = sum_c conj(Y[i][c]) Y[j][c]  (MFMA, wave w -> tile (w>>1, w&1)) ----
-  const int ti = wave >> 1, tj = wave & 1;
-  mfma_d4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
-  load_chunk(0, C);
-  for (int c0 = 0; c0 < C; c0 += JC) {
-    store_chunk();
-    __syncthreads();
-    if (c0 + JC < C) load_chunk(c0 + JC, C);
-#pragma unroll
-    for (int kk = 0; kk < JC / 4; ++kk) {
-      const double2 a = sY[ti * 16 + l16][kk * 4 + k4];
-      const double2 b = sY[tj * 16 + l16][kk * 4 + k4];
-      gr = mfma64(a.x, b.x, gr);
-      gr = mfma64(a.y, b.y, gr);
-      gi = mfma64(a.x, b.y, gi);
-      gi = mfma64(-a.y, b.x, gi);
+      for (int v = 0; v < 4; ++v) {
+        const int i = 4 * v + k4, j = l16;
+        double2 g00 = make_double2(a00r[v], a00i[v]);
+        double2 g01 = make_double2(a01r[v], a01i[v]);
+        double2 g11 = make_double2(a11r[v], a11i[v]);
+        if (w) {
+          g00 = cadd(g00, sG[i][j]);
+          g01 = cadd(g01, sG[i][JB + j]);
+          g11 = cadd(g11, sG[JB + i][JB + j]);
+        }
+        sG[i][j] = g00;
+        sG[i][JB + j] = g01;
+        sG[JB + i][JB + j] = g11;
+      }
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int v = 0; v < 4; ++v) sG[ti * 16 + 4 * v + k4][tj * 16 + l16] = make_double2(gr[v], gi[v]);
+  {
+    const int i = tid >> 4, j = tid & 15;
+    sG[JB + i][j] = cconj(sG[j][JB + i]);
+  }
   __syncthreads();
-
-  // ---- convergence test of this block pair ----
-  auto max_offdiag = [&]() -> double {
+  // ---- convergence test of this block pair: max |G_ij|^2 / (G_ii G_jj) over i != j ----
+  {
     double m = 0.0;
     for (int e = tid; e < JP * JP; e += 256) {
       int i = e / JP, j = e % JP;
       if (i == j) continue;
       double aii = sG[i][i].x, ajj = sG[j][j].x;
-      if (aii > 0.0 && ajj > 0.0) m = fmax(m, sqrt(cabs2(sG[i][j]) / (aii * ajj)));
+      if (aii > 0.0 && ajj > 0.0) m = fmax(m, cabs2(sG[i][j]) / (aii * ajj));
     }
     s_red[tid] = m;
     __syncthreads();
@@ -353,91 +407,140 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
       if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
       __syncthreads();
     }
-    double r = s_red[0];
-    __syncthreads();
-    return r;
-  };
+  }
   if (tid == 0) atomicAdd(T.visited, 1);
-  if (max_offdiag() <= tol) return;
+  if (s_red[0] <= tol * tol) return;
 
-  // ---- eigen-decomposition of the Gram block (cyclic two-sided Jacobi, FP64) ----
+  // ---- apply prefetch: this wave's first column tile, all 32 rows ----
+  const int nt = (CT + 15) >> 4;
+  double2 bq[8];
+  auto ld_tile = [&](int ctile, double2* b) {
+    const int col = ctile * 16 + l16;
+    const bool ok = ctile < nt && col < CT;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int row = kk * 4 + k4;
+      const double2* rp = Y + (size_t)(row < JB ? ba * JB + row : bb * JB + row - JB) * ld;
+      b[kk] = ok ? rp[col] : make_double2(0, 0);
+    }
+  };
+  ld_tile(wave, bq);
+
   for (int e = tid; e < JP * JP; e += 256) sW[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
+  const double itol = tol * 0.25;
+  if (tid < JP / 2) {
+    const int p = circle_slot(tid, 0, JP), q = circle_slot(JP - 1 - tid, 0, JP);
+    double c;
+    double2 se;
+    jacobi_rotation(sG[p][p].x, sG[q][q].x, sG[p][q], itol, c, se);
+    s_c[0][tid] = c;
+    s_se[0][tid] = se;
+  }
   __syncthreads();
-  for (int sw = 0; sw < inner_sweeps; ++sw) {
-    for (int step = 0; step < JP - 1; ++step) {
-      if (tid < JP / 2) {
-        int p = circle_slot(tid, step, JP), q = circle_slot(JP - 1 - tid, step, JP);
-        double c;
-        double2 se;
-        jacobi_rotation(sG[p][p].x, sG[q][q].x, sG[p][q], tol * 0.25, c, se);
-        s_p[tid] = p;
-        s_q[tid] = q;
-        s_c[tid] = c;
-        s_se[tid] = se;
-      }
-      __syncthreads();
-      // columns of G and W: x_p' = c x_p - conj(se) x_q ; x_q' = se x_p + c x_q
-      for (int u = tid; u < (JP / 2) * JP; u += 256) {
-        int k = u / JP, i = u % JP;
-        double c = s_c[k];
-        double2 se = s_se[k];
-        if (se.x == 0.0 && se.y == 0.0) continue;
-        int p = s_p[k], q = s_q[k];
-        double2 x = sG[i][p], y = sG[i][q];
-        sG[i][p] = cadd(cscale(x, c), cscale(cmul(cconj(se), y), -1.0));
-        sG[i][q] = cadd(cmul(se, x), cscale(y, c));
-        x = sW[i][p];
-        y = sW[i][q];
-        sW[i][p] = cadd(cscale(x, c), cscale(cmul(cconj(se), y), -1.0));
-        sW[i][q] = cadd(cmul(se, x), cscale(y, c));
-      }
-      __syncthreads();
-      // rows of G: G[p] = c G[p] - se G[q] ; G[q] = conj(se) G[p] + c G[q]
-      for (int u = tid; u < (JP / 2) * JP; u += 256) {
-        int k = u / JP, j = u % JP;
-        double c = s_c[k];
-        double2 se = s_se[k];
-        if (se.x == 0.0 && se.y == 0.0) continue;
-        int p = s_p[k], q = s_q[k];
-        double2 x = sG[p][j], y = sG[q][j];
-        sG[p][j] = cadd(cscale(x, c), cscale(cmul(se, y), -1.0));
-        sG[q][j] = cadd(cmul(cconj(se), x), cscale(y, c));
-      }
-      __syncthreads();
-    }
-    if (sw + 1 < inner_sweeps && max_offdiag() <= tol * 0.25) break;
+  // ---- inner sweep (one barrier per step) ----
+  const int bk = tid >> 4, bl = tid & 15;  // G block of rotation pairs bk, bl
+  const int wi = tid >> 3, wl = tid & 7;   // W row wi, rotation pairs wl and wl + 8
+  auto nxt = [](int x) { return x == JP - 1 ? 1 : x + 1; };  // row index one step later
+  int pk = bk, qk = JP - 1 - bk, pl = bl, ql = JP - 1 - bl;
+  int pw0 = wl, qw0 = JP - 1 - wl, pw1 = wl + 8, qw1 = JP - 1 - (wl + 8);
+  // next-step pair owned by this thread (jn), and which row of pair bk (rra) and of pair
+  // bl (rrb) its members are
+  int jn = -1, rra = 0, rrb = 0;
+  if (bk == 0 && bl == 1) { jn = 0; rra = 0; rrb = 0; }
+  else if (bk >= 2 && bl == bk - 2) { jn = bk - 1; rra = 0; rrb = 1; }
+  else if (bk == 15 && bl == 14) { jn = 15; rra = 1; rrb = 1; }
+  const int nsteps = inner_sweeps * (JP - 1);
+  double2 (*gA)[JP + 1] = sG;
+  double2 (*gB)[JP + 1] = sG2;
+  const int slot = jn >= 0 ? jn : JP / 2;  // non-owners write a dummy slot
+  for (int st = 0; st < nsteps; ++st) {
+    const int buf = st & 1;
+    // ---- all LDS reads of the step ----
+    const double ck = s_c[buf][bk], cl = s_c[buf][bl];
+    const double2 sek = s_se[buf][bk], sel = s_se[buf][bl];
+    const double2 b00 = gA[pk][pl], b01 = gA[pk][ql], b10 = gA[qk][pl], b11 = gA[qk][ql];
+    const double c2 = s_c[buf][wl], c3 = s_c[buf][wl + 8];
+    const double2 se2 = s_se[buf][wl], se3 = s_se[buf][wl + 8];
+    const double2 w0p = sW[wi][pw0], w0q = sW[wi][qw0], w1p = sW[wi][pw1], w1q = sW[wi][qw1];
+    const double ak = gA[pk][pk].x, bk2 = gA[qk][qk].x, al2 = gA[pl][pl].x, bl2 = gA[ql][ql].x;
+    const double2 gk = gA[pk][qk], gl = gA[pl][ql];
+    // ---- block B' = Jk^H B Jl,  J = [[c, se], [-conj(se), c]] ----
+    const double2 nsel = make_double2(-sel.x, sel.y);
+    const double2 m00 = cfma(b01, nsel, cscale(b00, cl)), m01 = cfma(b00, sel, cscale(b01, cl));
+    const double2 m10 = cfma(b11, nsel, cscale(b10, cl)), m11 = cfma(b10, sel, cscale(b11, cl));
+    const double2 nsek = make_double2(-sek.x, -sek.y), csek = cconj(sek);
+    const double2 n00 = cfma(nsek, m10, cscale(m00, ck)), n01 = cfma(nsek, m11, cscale(m01, ck));
+    const double2 n10 = cfma(csek, m00, cscale(m10, ck)), n11 = cfma(csek, m01, cscale(m11, ck));
+    // ---- W' = W J on this thread's two pairs (identity rotations leave W unchanged) ----
+    const double2 v0p = cfma(w0q, make_double2(-se2.x, se2.y), cscale(w0p, c2));
+    const double2 v0q = cfma(w0p, se2, cscale(w0q, c2));
+    const double2 v1p = cfma(w1q, make_double2(-se3.x, se3.y), cscale(w1p, c3));
+    const double2 v1q = cfma(w1p, se3, cscale(w1q, c3));
+    // ---- the next step's rotation (kept by its owner only) ----
+    const double rk = 2.0 * ck * (gk.x * sek.x + gk.y * sek.y);
+    const double ek = cabs2(sek), c2k = ck * ck;
+    const double alpha = rra ? (ek * ak + c2k * bk2 + rk) : (c2k * ak + ek * bk2 - rk);
+    const double rl = 2.0 * cl * (gl.x * sel.x + gl.y * sel.y);
+    const double el = cabs2(sel), c2l = cl * cl;
+    const double beta = rrb ? (el * al2 + c2l * bl2 + rl) : (c2l * al2 + el * bl2 - rl);
+    const double2 gam = rra ? (rrb ? n11 : n10) : (rrb ? n01 : n00);
+    double cn;
+    double2 sen;
+    jacobi_rotation(alpha, beta, gam, itol, cn, sen);
+    // ---- all LDS writes of the step ----
+    gB[pk][pl] = n00;
+    gB[pk][ql] = n01;
+    gB[qk][pl] = n10;
+    gB[qk][ql] = n11;
+    sW[wi][pw0] = v0p;
+    sW[wi][qw0] = v0q;
+    sW[wi][pw1] = v1p;
+    sW[wi][qw1] = v1q;
+    s_c[buf ^ 1][slot] = cn;
+    s_se[buf ^ 1][slot] = sen;
+    __syncthreads();
+    double2 (*tmp)[JP + 1] = gA;
+    gA = gB;
+    gB = tmp;
+    if (bk) pk = nxt(pk);
+    qk = nxt(qk);
+    if (bl) pl = nxt(pl);
+    ql = nxt(ql);
+    if (wl) pw0 = nxt(pw0);
+    qw0 = nxt(qw0);
+    pw1 = nxt(pw1);
+    qw1 = nxt(qw1);
   }
 
-  // ---- apply  Y'[i][c] = sum_j W[j][i] Y[j][c]  over data and rotation columns (MFMA) ----
-  // wave w -> output rows (w & 1)*16.., output column tile (w >> 1) of each 32-column chunk
-  const int ri = wave & 1, ct = wave >> 1;
-  double ar[JP / 4], ai[JP / 4];
+  // ---- apply Y' = W^T Y; wave owns column tiles wave, wave + 4, ... (all 32 rows) ----
+  for (int ctile = wave; ctile < nt; ctile += 4) {
+    mfma_d4 r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
+    const int ncol = (ctile + 4) * 16 + l16;
+    const bool nok = ctile + 4 < nt && ncol < CT;
 #pragma unroll
-  for (int kk = 0; kk < JP / 4; ++kk) {
-    double2 w = sW[kk * 4 + k4][ri * 16 + l16];  // A[i][k] = W[k][i]
-    ar[kk] = w.x;
-    ai[kk] = w.y;
-  }
-  load_chunk(0, CT);
-  for (int c0 = 0; c0 < CT; c0 += JC) {
-    store_chunk();
-    __syncthreads();
-    if (c0 + JC < CT) load_chunk(c0 + JC, CT);
-    mfma_d4 dr = {0, 0, 0, 0}, di = {0, 0, 0, 0};
-#pragma unroll
-    for (int kk = 0; kk < JP / 4; ++kk) {
-      const double2 b = sY[kk * 4 + k4][ct * 16 + l16];
-      dr = mfma64(ar[kk], b.x, dr);
-      dr = mfma64(-ai[kk], b.y, dr);
-      di = mfma64(ar[kk], b.y, di);
-      di = mfma64(ai[kk], b.x, di);
+    for (int kk = 0; kk < 8; ++kk) {
+      const double2 b = bq[kk];
+      const double2 w0 = sW[kk * 4 + k4][l16];       // A[i][k] = W[k][i], rows 0..15
+      const double2 w1 = sW[kk * 4 + k4][16 + l16];  // rows 16..31
+      const int row = kk * 4 + k4;
+      bq[kk] = nok ? Y[(size_t)(row < JB ? ba * JB + row : bb * JB + row - JB) * ld + ncol] : make_double2(0, 0);
+      r0 = mfma64(w0.x, b.x, r0);
+      r1 = mfma64(w1.x, b.x, r1);
+      i0 = mfma64(w0.x, b.y, i0);
+      i1 = mfma64(w1.x, b.y, i1);
+      r0 = mfma64(-w0.y, b.y, r0);
+      r1 = mfma64(-w1.y, b.y, r1);
+      i0 = mfma64(w0.y, b.x, i0);
+      i1 = mfma64(w1.y, b.x, i1);
     }
-    const int col = c0 + ct * 16 + l16;
+    const int col = ctile * 16 + l16;
     if (col < CT) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Y[(size_t)grow(ri * 16 + 4 * v + k4) * ld + col] = make_double2(dr[v], di[v]);
+      for (int v = 0; v < 4; ++v) {
+        Y[(size_t)(ba * JB + 4 * v + k4) * ld + col] = make_double2(r0[v], i0[v]);
+        Y[(size_t)(bb * JB + 4 * v + k4) * ld + col] = make_double2(r1[v], i1[v]);
+      }
     }
-    __syncthreads();
   }
   if (tid == 0) atomicAdd(T.counter, 1);
 }
@@ -445,7 +548,7 @@ __global__ __launch_bounds__(256) void k_jacobi_round(const JacTask* tasks, int 
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
                                int inner_sweeps, hipStream_t s) {
   if (total_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_round, dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round, inner_sweeps);
+  hipLaunchKernelGGL(k_jacobi_round<JB>, dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round, inner_sweeps);
   return hipGetLastError();
 }
 
