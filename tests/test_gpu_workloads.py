@@ -115,3 +115,20 @@ def test_mixed_batch_vs_oracle_mps(eng):
             assert abs(v - m.expval_z_parity([a, b])) < TOL64
         assert abs(r["discarded_weight"] - m.discarded()) < 1e-10
         assert r["max_bond"] == max(m.bonds())
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL64), ("mixed", TOLMIX)])
+def test_sycamore_subcircuit_vs_dense(eng, precision, tol):
+    """qFlex ingest end to end: the reference's Sycamore depth-10 circuit file, restricted
+    to the gates on the first 14 line qubits, routed, against the dense state vector."""
+    import os
+    say("sycamore", precision)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sycamore", "sycamore_53_10_0.txt")
+    gates = C.restrict(C.read_qflex(open(path).read()), 14)
+    ref = O.StateVector(14).run(gates)
+    s = mps.MpsState(eng, 14, cutoff=0.0, precision=precision).run(C.route_nearest_neighbour(gates))
+    ez = s.expval_z_all()
+    for q in range(14):
+        assert abs(ez[q] - ref.expval_z_parity([q])) < tol
+    bits = C.random_bitstrings(14, 16, seed=11)
+    assert np.abs(s.amplitudes(bits) - ref.amplitudes(bits)).max() < tol

@@ -146,3 +146,46 @@ def test_mixed_batch_ranges_and_seed():
         assert (x["n"], x["depth"], x["chi"]) == (y["n"], y["depth"], y["chi"])
         assert 16 <= x["n"] <= 48 and 10 <= x["depth"] <= 60 and x["chi"] in (32, 64, 128, 256)
         assert C.count_2q(x["gates"]) == sum(len(range(l % 2, x["n"] - 1, 2)) for l in range(x["depth"]))
+
+
+# ------------------------------------------------------------------ qFlex ingest
+import os
+import re
+
+SYC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sycamore")
+
+
+def _read_xasm(path):
+    """Gate list of the reference converter's XASM output (fixture)."""
+    out = []
+    pat = re.compile(r"^(\w+)\(q\[(\d+)\](?:, q\[(\d+)\])?((?:, [-+0-9.eE]+)*)\);")
+    for line in open(path):
+        m = pat.match(line.strip())
+        if not m:
+            continue
+        qs = (int(m.group(2)),) if m.group(3) is None else (int(m.group(2)), int(m.group(3)))
+        ps = tuple(float(x) for x in m.group(4).split(",")[1:]) if m.group(4) else ()
+        out.append({"name": m.group(1), "qubits": qs, "params": ps})
+    return out
+
+
+@pytest.mark.parametrize("conj,fixture", [(0, "sycamore_53_10_0.xasm"), (10, "sycamore_53_10_0_Conjugate.xasm")])
+def test_qflex_reader_matches_reference_converter(conj, fixture):
+    """The reference's own converter output (examples/sycamore/resources) is the golden
+    file: names, qubits and parameters must agree bit for bit."""
+    text = open(os.path.join(SYC, "sycamore_53_10_0.txt")).read()
+    mine = C.read_qflex(text, conjugate_last_layer=conj)
+    ref = _read_xasm(os.path.join(SYC, fixture))
+    assert len(mine) == len(ref)
+    for a, b in zip(mine, ref):
+        assert a["name"] == b["name"] and tuple(a["qubits"]) == b["qubits"], (a, b)
+        assert tuple(a["params"]) == b["params"], (a, b)
+
+
+def test_qflex_subcircuit_routed_equals_dense():
+    text = open(os.path.join(SYC, "sycamore_53_10_0.txt")).read()
+    gates = C.restrict(C.read_qflex(text), 12)
+    assert C.count_2q(gates) > 20
+    a = O.StateVector(12).run(gates)
+    b = O.StateVector(12).run(C.route_nearest_neighbour(gates))
+    assert np.abs(a.psi - b.psi).max() < 1e-12

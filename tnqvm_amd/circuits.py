@@ -11,6 +11,7 @@ Workloads (all seeded; the seed is recorded by the callers):
   * ``qaoa_maxcut``    QAOA on a random 3-regular graph routed by a swap network
   * ``grid_fsim``      6x6 grid fSim circuits, snake-mapped onto a line
   * ``mixed_batch``    many brickwork circuits of uneven width, depth and chi_max
+  * ``read_qflex``     qFlex circuit files (Google Sycamore random circuits)
 """
 from __future__ import annotations
 
@@ -252,3 +253,73 @@ def mixed_batch(count: int = 256, seed: int = 0x7E5A0004, n_range=(16, 48), dept
         chi = int(rng.choice(chis))
         out.append({"n": n, "depth": d, "chi": chi, "gates": haar_brickwork(n, d, seed=seed * 1000 + k)})
     return out
+
+
+# ------------------------------------------------------------------ qFlex circuit files
+# Sycamore lattice index -> position on the line (the converter's table,
+# examples/sycamore/parse_qflex.py:9-61).
+SYCAMORE_53 = dict(zip(
+    [5, 6, 14, 15, 16, 17, 24, 25, 26, 27, 28, 32, 33, 34, 35, 36, 37, 38, 39, 41, 42, 43, 44, 45, 46, 47, 48,
+     49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 61, 62, 63, 64, 65, 66, 67, 72, 73, 74, 75, 76, 83, 84, 85, 94],
+    range(53)))
+# qFlex gives angles as fractions of pi; the reference converter multiplies by this
+# truncated value (parse_qflex.py:75, :115-116), and the same constant is used here so that
+# the gate parameters are bit-identical to the XASM it produced.
+QFLEX_PI = 3.14159265359
+_HALF_PI = 1.57079632679
+_QUARTER_PI = 0.78539816339
+
+
+def _qflex_line_gates(line: str, qubit_map, conjugate: bool):
+    """Gates of one qFlex line "<cycle> <gate> <qubit> [<qubit>]" (parse_qflex.py:65-124)."""
+    parts = line.replace(", ", ",").split()
+    if len(parts) == 3:
+        _, name, q = parts
+        q = qubit_map[int(q)]
+        sign = -1.0 if conjugate else 1.0
+        if name == "hz_1_2":
+            # Cirq PhasedXPowGate(0.25, 0.5) as Rz(-pi/4) Rx(pi/2) Rz(pi/4) in time order; the
+            # converter's conjugate form only negates the Rx angle (parse_qflex.py:83-98)
+            return [{"name": "Rz", "qubits": (q,), "params": (-_QUARTER_PI,)},
+                    {"name": "Rx", "qubits": (q,), "params": (sign * _HALF_PI,)},
+                    {"name": "Rz", "qubits": (q,), "params": (_QUARTER_PI,)}]
+        if name == "x_1_2":
+            return [{"name": "Rx", "qubits": (q,), "params": (sign * _HALF_PI,)}]
+        if name == "y_1_2":
+            return [{"name": "Ry", "qubits": (q,), "params": (sign * _HALF_PI,)}]
+        if name.lower().startswith("rz("):
+            return [{"name": "Rz", "qubits": (q,), "params": (sign * (float(name[3:-1]) * QFLEX_PI),)}]
+        raise ValueError(f"unknown qFlex 1-qubit gate {name!r}")
+    if len(parts) == 4:
+        _, name, q0, q1 = parts
+        if not name.startswith("fsim("):
+            raise ValueError(f"unknown qFlex 2-qubit gate {name!r}")
+        th, ph = (float(x) * QFLEX_PI for x in name[5:-1].split(","))
+        if conjugate:
+            th, ph = -th, -ph
+        return [{"name": "fSim", "qubits": (qubit_map[int(q0)], qubit_map[int(q1)]), "params": (th, ph)}]
+    raise ValueError(f"malformed qFlex line {line!r}")
+
+
+def read_qflex(text: str, qubit_map=None, conjugate_last_layer: int = 0):
+    """Gate list of a qFlex circuit file (first line: qubit count; then one gate per line,
+    "<cycle> <gate> <qubits>").  Qubits are placed on the line by `qubit_map` (default:
+    the 53-qubit Sycamore map).  With conjugate_last_layer = L > 0, the gates of layer L-1
+    (cycles 4(L-1)+1 .. 4L) are appended again, reversed and conjugated, as the reference
+    converter does for its debug circuits (parse_qflex.py:126-195)."""
+    qubit_map = SYCAMORE_53 if qubit_map is None else qubit_map
+    lines = [ln for ln in text.splitlines() if ln.strip()]
+    gates, last = [], []
+    for ln in lines[1:]:
+        cycle = int(ln.split()[0])
+        gates.extend(_qflex_line_gates(ln, qubit_map, False))
+        if conjugate_last_layer and cycle > 0 and (cycle - 1) // 4 == conjugate_last_layer - 1:
+            last.append(ln)
+    for ln in reversed(last):
+        gates.extend(_qflex_line_gates(ln, qubit_map, True))
+    return gates
+
+
+def restrict(gates, keep: int):
+    """The gates that act only on qubits < keep (a sub-circuit small enough for dense checks)."""
+    return [g for g in gates if max(g["qubits"]) < keep]
