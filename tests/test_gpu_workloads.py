@@ -132,3 +132,18 @@ def test_sycamore_subcircuit_vs_dense(eng, precision, tol):
         assert abs(ez[q] - ref.expval_z_parity([q])) < tol
     bits = C.random_bitstrings(14, 16, seed=11)
     assert np.abs(s.amplitudes(bits) - ref.amplitudes(bits)).max() < tol
+
+
+def test_brickwork_in_reference_gate_set(eng):
+    """A Haar brickwork emitted as U3 + CNOT (KAK) gives the same <Z> and |amplitudes| on
+    hip-mps as the dense run of the original 4x4 unitaries."""
+    n = 12
+    say("kak brickwork", n)
+    gates = C.haar_brickwork(n, 10, seed=21)
+    ref = O.StateVector(n).run(gates)
+    s = mps.MpsState(eng, n, cutoff=0.0).run(C.to_reference_gates(gates))
+    ez = s.expval_z_all()
+    for q in range(n):
+        assert abs(ez[q] - ref.expval_z_parity([q])) < TOL64
+    bits = C.random_bitstrings(n, 16, seed=4)
+    assert np.abs(np.abs(s.amplitudes(bits)) - np.abs(ref.amplitudes(bits))).max() < TOL64

@@ -189,3 +189,54 @@ def test_qflex_subcircuit_routed_equals_dense():
     a = O.StateVector(12).run(gates)
     b = O.StateVector(12).run(C.route_nearest_neighbour(gates))
     assert np.abs(a.psi - b.psi).max() < 1e-12
+
+
+# ------------------------------------------------------------------ KAK emission
+def _matrix_of_2q(gates, n=2):
+    """4x4 matrix (first qubit = MSB) of a gate list on qubits 0, 1 through the dense oracle."""
+    cols = []
+    for k in range(4):
+        s = O.StateVector(n)
+        s.psi[:] = 0
+        s.psi[(k >> 1) | ((k & 1) << 1)] = 1
+        s.run(gates)
+        cols.append([s.psi[(j >> 1) | ((j & 1) << 1)] for j in range(4)])
+    return np.array(cols).T
+
+
+def _equal_up_to_phase(a, b, tol):
+    k = np.argmax(np.abs(b))
+    ph = a.flat[k] / b.flat[k]
+    return abs(abs(ph) - 1) < tol and np.abs(a - ph * b).max() < tol
+
+
+def test_kak_emission_reconstructs_haar_and_named_gates():
+    rng = np.random.default_rng(17)
+    mats = [C.haar_unitary(rng, 4) for _ in range(60)]
+    mats += [O.gate_matrix(n, p) for n, p in [("CNOT", ()), ("CZ", ()), ("Swap", ()), ("iSwap", ()),
+                                              ("fSim", (0.3, 0.7)), ("CPhase", (0.4,))]]
+    mats += [np.eye(4), np.kron(C.haar_unitary(rng, 2), C.haar_unitary(rng, 2))]
+    for u in mats:
+        g = C.kak_emit(u, 0, 1)
+        assert sum(x["name"] == "CNOT" for x in g) == 3 and sum(x["name"] == "U" for x in g) == 8
+        assert _equal_up_to_phase(_matrix_of_2q(g), u, 1e-12)
+
+
+def test_kak_coordinates_of_known_gates():
+    """CNOT is locally exp(i pi/4 XX) (one coordinate pi/4, two zero); SWAP has all three pi/4."""
+    _, abc, _ = C.kak(O.gate_matrix("CNOT", ()))
+    assert sorted(np.round(np.abs(abc), 12)) == [0, 0, round(np.pi / 4, 12)]
+    _, abc, _ = C.kak(O.gate_matrix("Swap", ()))
+    assert np.allclose(np.abs(abc), np.pi / 4, atol=1e-12)
+
+
+def test_reference_gate_brickwork_equals_unitary_brickwork():
+    n = 8
+    gates = C.haar_brickwork(n, 6, seed=3)
+    ref = C.to_reference_gates(gates)
+    assert all(g["name"] in ("U", "CNOT") for g in ref)
+    a = O.StateVector(n).run(gates)
+    b = O.StateVector(n).run(ref)
+    assert abs(abs(np.vdot(a.psi, b.psi)) - 1) < 1e-12
+    for q in range(n):
+        assert abs(a.expval_z_parity([q]) - b.expval_z_parity([q])) < 1e-12

@@ -12,6 +12,7 @@ Workloads (all seeded; the seed is recorded by the callers):
   * ``grid_fsim``      6x6 grid fSim circuits, snake-mapped onto a line
   * ``mixed_batch``    many brickwork circuits of uneven width, depth and chi_max
   * ``read_qflex``     qFlex circuit files (Google Sycamore random circuits)
+  * ``to_reference_gates``  explicit unitaries as U3 / CNOT (KAK: 3 CNOT + 8 U3 per 2q gate)
 """
 from __future__ import annotations
 
@@ -323,3 +324,104 @@ def read_qflex(text: str, qubit_map=None, conjugate_last_layer: int = 0):
 def restrict(gates, keep: int):
     """The gates that act only on qubits < keep (a sub-circuit small enough for dense checks)."""
     return [g for g in gates if max(g["qubits"]) < keep]
+
+
+# ------------------------------------------------------------------ KAK emission (3 CNOT + 8 U3)
+# Magic basis: B^H (A x C) B is real orthogonal for A, C in SU(2), and XX, YY, ZZ are
+# diagonal in it (Kraus & Cirac, PRA 63, 062309 (2001); Vatan & Williams, PRA 69,
+# 032315 (2004)).
+_MAGIC = np.array([[1, 0, 0, 1j], [0, 1j, 1, 0], [0, 1j, -1, 0], [1, 0, 0, -1j]]) / np.sqrt(2.0)
+_PX = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+_PY = np.array([[0, -1j], [1j, 0]], dtype=np.complex128)
+_PZ = np.diag([1.0, -1.0]).astype(np.complex128)
+
+
+def _rz(t):
+    return np.diag([np.exp(-0.5j * t), np.exp(0.5j * t)])
+
+
+def _ry(t):
+    c, s = np.cos(0.5 * t), np.sin(0.5 * t)
+    return np.array([[c, -s], [s, c]], dtype=np.complex128)
+
+
+def u3_params(v: np.ndarray):
+    """(theta, phi, lambda) with v = e^{i g} U3(theta, phi, lambda), U3 as in Gates.hpp:224-235:
+    [[cos t/2, -e^{i l} sin t/2], [e^{i p} sin t/2, e^{i(p+l)} cos t/2]]."""
+    a00, a01, a10, a11 = v[0, 0], v[0, 1], v[1, 0], v[1, 1]
+    theta = 2.0 * np.arctan2(abs(a10), abs(a00))
+    g = np.angle(a00) if abs(a00) > 1e-12 else np.angle(a10)
+    phi = np.angle(a10) - g if abs(a10) > 1e-12 else 0.0
+    lam = np.angle(-a01) - g if abs(a01) > 1e-12 else np.angle(a11) - g - phi
+    return float(theta), float(phi), float(lam)
+
+
+def _split_local(k: np.ndarray):
+    """A1, A2 with k = A1 x A2 (rank-1 factor of the realigned 4x4)."""
+    r = k.reshape(2, 2, 2, 2).transpose(0, 2, 1, 3).reshape(4, 4)
+    u, s, vh = np.linalg.svd(r)
+    return np.sqrt(s[0]) * u[:, 0].reshape(2, 2), np.sqrt(s[0]) * vh[0].reshape(2, 2)
+
+
+def kak(u: np.ndarray):
+    """u = phase (A1 x A2) exp(i(a XX + b YY + c ZZ)) (B1 x B2); returns ((A1, A2), (a, b, c), (B1, B2))."""
+    u = np.asarray(u, dtype=np.complex128)
+    u = u / np.linalg.det(u) ** 0.25
+    um = _MAGIC.conj().T @ u @ _MAGIC
+    m2 = um.T @ um  # symmetric unitary; Re and Im commute and share a real eigenbasis
+    rng = np.random.default_rng(0x7E5A)
+    for _ in range(32):
+        _, p = np.linalg.eigh(m2.real + rng.standard_normal() * m2.imag)
+        d = p.T @ m2 @ p
+        if np.abs(d - np.diag(np.diag(d))).max() < 1e-9:
+            break
+    else:
+        raise RuntimeError("kak: no common real eigenbasis")
+    if np.linalg.det(p) < 0:
+        p[:, 0] *= -1.0
+    th = 0.5 * np.angle(np.diag(p.T @ m2 @ p))
+    if np.prod(np.exp(1j * th)).real < 0:
+        th[0] += np.pi
+    o1 = um @ p @ np.diag(np.exp(-1j * th))
+    if np.abs(o1.imag).max() > 1e-7:
+        raise RuntimeError("kak: left factor is not real")
+    k1 = _MAGIC @ o1.real @ _MAGIC.conj().T
+    k2 = _MAGIC @ p.T @ _MAGIC.conj().T
+    basis = np.stack([np.diag(_MAGIC.conj().T @ np.kron(m, m) @ _MAGIC).real for m in (_PX, _PY, _PZ)]
+                     + [np.ones(4)], axis=1)
+    a, b, c, _ = np.linalg.solve(basis, th)
+    return _split_local(k1), (float(a), float(b), float(c)), _split_local(k2)
+
+
+def kak_emit(u: np.ndarray, q0: int, q1: int):
+    """Gates (U3 and CNOT, in time order) equal to the 4x4 u on (q0, q1), q0 = most significant
+    bit of u's index, up to a global phase.  exp(i(aXX + bYY + cZZ)) is emitted as
+    (I x Rz(pi/2)), CNOT(q1,q0), (Rz(pi/2-2c) x Ry(pi/2-2a)), CNOT(q0,q1), (I x Ry(2b-pi/2)),
+    CNOT(q1,q0), (Rz(-pi/2) x I) (Vatan & Williams, Fig. 6), and the one-qubit factors are merged
+    into the surrounding U3s: 3 CNOT + 8 U3."""
+    (a1, a2), (a, b, c), (b1, b2) = kak(u)
+    seq = [(q0, b1), (q1, _rz(np.pi / 2) @ b2), ("CNOT", q1, q0),
+           (q0, _rz(np.pi / 2 - 2 * c)), (q1, _ry(np.pi / 2 - 2 * a)), ("CNOT", q0, q1),
+           (q0, np.eye(2)), (q1, _ry(2 * b - np.pi / 2)), ("CNOT", q1, q0),
+           (q0, a1 @ _rz(-np.pi / 2)), (q1, a2)]
+    out = []
+    for it in seq:
+        if it[0] == "CNOT":
+            out.append({"name": "CNOT", "qubits": (it[1], it[2]), "params": ()})
+        else:
+            out.append({"name": "U", "qubits": (it[0],), "params": u3_params(it[1])})
+    return out
+
+
+def to_reference_gates(gates):
+    """Explicit unitaries rewritten in the reference gate set: 1q -> U3, 2q -> 3 CNOT + 8 U3.
+    Each gate is reproduced up to a global phase."""
+    out = []
+    for g in gates:
+        if g["name"] != "unitary":
+            out.append(g)
+        elif len(g["qubits"]) == 1:
+            out.append({"name": "U", "qubits": g["qubits"], "params": u3_params(np.asarray(g["matrix"]))})
+        else:
+            out.extend(kak_emit(g["matrix"], *g["qubits"]))
+    return out
