@@ -13,7 +13,7 @@
 // Execution model.  Every 2-qubit gate becomes a task of a "wave": a set of gates on
 // pairwise disjoint site pairs, possibly from many circuits.  One wave runs as a
 // fixed sequence of batched launches on the engine stream:
-//   k_theta -> k_jacobi_prep -> (k_jacobi_round x rounds) x sweeps -> k_rownorm ->
+//   k_theta_mfma -> k_jacobi_prep -> (k_jacobi_round x rounds) x sweeps -> k_rownorm ->
 //   k_select -> k_gather + k_gemm
 // so the device sees all independent SVDs of the wave at once.  1-qubit gates are
 // fused into the next 2-qubit gate on the same qubit (U_2q . (U_a x U_b)); whatever is

@@ -2,7 +2,7 @@
 //
 // Kernel map (reference function each one replaces, tnqvm/visitors/exatn-mps/):
 //   k_apply_1q ......... 1-qubit gate into a site      ExaTnMpsVisitor.cpp:1231-1273
-//   k_theta ............ merge + 2-qubit gate          ExaTnMpsVisitor.cpp:1401-1541
+//   k_theta_mfma ....... merge + 2-qubit gate          ExaTnMpsVisitor.cpp:1401-1541
 //                        (+ stabilizeTensorBody, :141-161)
 //   k_jacobi_prep,
 //   k_jacobi_round ..... full SVD (one-sided block Jacobi, FP64)  :1555-1627
@@ -98,14 +98,54 @@ hipError_t launch_apply_1q(void* site, int fp32, int cl, int cr, const double2 m
 }
 
 // ------------------------------------------------------------------ theta = G (Ai Aj)
-// Tile: 16 values of l (32 rows (l,a)) x 32 values of r (64 columns (b,r)).
-// Thread: one l, two r (r, r+16), all four (a,b): 8 complex accumulators.
-constexpr int TH_L = 16, TH_R = 32, TH_K = 16;
+// Workgroup tile: 16 values of l (32 rows (l,a)) x 32 values of r (64 columns (b,r)).
+constexpr int TH_L = 16, TH_R = 32;
+
+// ------------------------------------------------------------------ theta on MFMA
+// theta'[(l,a)][(b,r)] = sum_{a'b'} G[(a,b),(a',b')] sum_m Ai[l][a'][m] Aj[m][b'][r] as a complex
+// GEMM on the matrix cores: complex64 sites use v_mfma_f32_16x16x4f32 (FP32 products and FP32
+// accumulation), complex128 sites v_mfma_f64_16x16x4f64; each complex product is four real
+// MFMAs.  Workgroup tile: 16 l x 32 r; wave w: 8 l x 16 r, i.e. one 16-row MFMA
+// tile (rows = 8 l x 2 a) against two 16-column tiles (b = 0, 1).  The MFMA rows are ordered
+// so that the a = 0 and a = 1 rows of one l land in the same lane, as do b = 0 and b = 1, so
+// the 4x4 gate mixes the (a, b) block of each (l, r) in registers:
+//   f32 D[i][j]: lane j + 16 (i / 4), register i % 4  -> row i = 4g + v: l = 2g + (v & 1), a = v >> 1
+//   f64 D[i][j]: lane j + 16 (i % 4), register i / 4  -> row i = 4v + g: l = 2g + (v & 1), a = v >> 1
+template <typename T>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<float2> {
+  typedef float scalar;
+  typedef float acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // (l offset, a) of MFMA row i
+  static __device__ __forceinline__ int row_l(int i) { return 2 * (i >> 2) + (i & 1); }
+  static __device__ __forceinline__ int row_a(int i) { return (i >> 1) & 1; }
+  // (lane group g, register v) holding D[i][.]
+  static __device__ __forceinline__ int out_row(int g, int v) { return 4 * g + v; }
+};
+template <>
+struct MfmaTraits<double2> {
+  typedef double scalar;
+  typedef double acc __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row_l(int i) { return 2 * (i & 3) + ((i >> 2) & 1); }
+  static __device__ __forceinline__ int row_a(int i) { return i >> 3; }
+  static __device__ __forceinline__ int out_row(int g, int v) { return 4 * v + g; }
+};
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_theta(const ThetaTask* tasks, int ntasks) {
-  __shared__ double2 sAi[TH_L][2][TH_K + 1];
-  __shared__ double2 sAj[TH_K][2][TH_R + 1];
+__global__ __launch_bounds__(256) void k_theta_mfma(const ThetaTask* tasks, int ntasks) {
+  typedef MfmaTraits<T> M;
+  typedef typename M::scalar S;
+  typedef typename M::acc A4;
+  constexpr int KC = 16;  // contraction chunk staged in LDS
+  __shared__ T sAi[TH_L][2][KC + 1];
+  __shared__ T sAj[KC][2][TH_R + 1];
   const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const ThetaTask& x) { return x.tile_begin; });
   const ThetaTask& task = tasks[t];
   const int tile = blockIdx.x - task.tile_begin;
@@ -115,57 +155,63 @@ __global__ __launch_bounds__(256) void k_theta(const ThetaTask* tasks, int ntask
   const T* Ai = (const T*)task.Ai;
   const T* Aj = (const T*)task.Aj;
   const int tid = threadIdx.x;
-  const int tl = tid / 16, tr = tid % 16;
-  double2 acc[2][4];
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) acc[x][y] = make_double2(0, 0);
-
-  for (int m0 = 0; m0 < cm; m0 += TH_K) {
+  const int wave = tid >> 6, lane = tid & 63;
+  const int l16 = lane & 15, k4 = lane >> 4;
+  const int wl = (wave >> 1) * 8, wr = (wave & 1) * 16;  // this wave's l and r offsets in the tile
+  const int my_l = wl + M::row_l(l16), my_a = M::row_a(l16);  // A operand row of this lane
+  A4 re0 = {0, 0, 0, 0}, im0 = {0, 0, 0, 0}, re1 = {0, 0, 0, 0}, im1 = {0, 0, 0, 0};
+  for (int m0 = 0; m0 < cm; m0 += KC) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      int e = tid + 256 * q;  // 512 = 16 l x 2 a x 16 k
-      int l = e / 32, a = (e / 16) % 2, k = e % 16;
-      double2 v = make_double2(0, 0);
-      if (l0 + l < cl && m0 + k < cm) v = ld(Ai, ((size_t)(l0 + l) * 2 + a) * cm + m0 + k);
+      const int e = tid + 256 * q;  // 512 = 16 l x 2 a x 16 k
+      const int l = e >> 5, a = (e >> 4) & 1, k = e & 15;
+      T v = T{0, 0};
+      if (l0 + l < cl && m0 + k < cm) v = Ai[((size_t)(l0 + l) * 2 + a) * cm + m0 + k];
       sAi[l][a][k] = v;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int e = tid + 256 * q;  // 1024 = 16 k x 2 b x 32 r
-      int k = e / 64, b = (e / 32) % 2, rr = e % 32;
-      double2 v = make_double2(0, 0);
-      if (m0 + k < cm && r0 + rr < cr) v = ld(Aj, ((size_t)(m0 + k) * 2 + b) * cr + r0 + rr);
+      const int e = tid + 256 * q;  // 1024 = 16 k x 2 b x 32 r
+      const int k = e >> 6, b = (e >> 5) & 1, rr = e & 31;
+      T v = T{0, 0};
+      if (m0 + k < cm && r0 + rr < cr) v = Aj[((size_t)(m0 + k) * 2 + b) * cr + r0 + rr];
       sAj[k][b][rr] = v;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < TH_K; ++k) {
-      double2 a0 = sAi[tl][0][k], a1 = sAi[tl][1][k];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        double2 b0 = sAj[k][0][tr + 16 * x], b1 = sAj[k][1][tr + 16 * x];
-        acc[x][0] = cfma(a0, b0, acc[x][0]);
-        acc[x][1] = cfma(a0, b1, acc[x][1]);
-        acc[x][2] = cfma(a1, b0, acc[x][2]);
-        acc[x][3] = cfma(a1, b1, acc[x][3]);
-      }
+    for (int kk = 0; kk < KC / 4; ++kk) {
+      const T x = sAi[my_l][my_a][kk * 4 + k4];            // A[i][k], i = l16
+      const T y0 = sAj[kk * 4 + k4][0][wr + l16];          // B[k][j], b = 0
+      const T y1 = sAj[kk * 4 + k4][1][wr + l16];          // b = 1
+      const S xr = x.x, xi = x.y, nxi = -x.y;
+      re0 = M::mma(xr, y0.x, re0);
+      re0 = M::mma(nxi, y0.y, re0);
+      im0 = M::mma(xr, y0.y, im0);
+      im0 = M::mma(xi, y0.x, im0);
+      re1 = M::mma(xr, y1.x, re1);
+      re1 = M::mma(nxi, y1.y, re1);
+      im1 = M::mma(xr, y1.y, im1);
+      im1 = M::mma(xi, y1.x, im1);
     }
     __syncthreads();
   }
-  const int l = l0 + tl;
-  if (l >= cl) return;
+  // registers v and v + 2 of this lane hold a = 0, 1 of one l; re0/im0 b = 0, re1/im1 b = 1
+  const int r = r0 + wr + l16;
   const size_t ldt = (size_t)2 * cr;
 #pragma unroll
-  for (int x = 0; x < 2; ++x) {
-    int r = r0 + tr + 16 * x;
-    if (r >= cr) continue;
+  for (int v = 0; v < 2; ++v) {
+    const int l = l0 + wl + M::row_l(M::out_row(k4, v));
+    if (l >= cl || r >= cr) continue;
+    double2 x[4];  // (a', b') = (0,0), (0,1), (1,0), (1,1)
+    x[0] = make_double2(re0[v], im0[v]);
+    x[1] = make_double2(re1[v], im1[v]);
+    x[2] = make_double2(re0[v + 2], im0[v + 2]);
+    x[3] = make_double2(re1[v + 2], im1[v + 2]);
 #pragma unroll
     for (int ab = 0; ab < 4; ++ab) {
       double2 o = make_double2(0, 0);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) o = cfma(task.G[ab * 4 + k], acc[x][k], o);
+      for (int k = 0; k < 4; ++k) o = cfma(task.G[ab * 4 + k], x[k], o);
       // stabilizeTensorBody: |x| < 1e-100 -> 0 (ExaTnMpsVisitor.cpp:141-161)
       if (cabs2(o) < 1e-200) o = make_double2(0, 0);
       const int a = ab >> 1, b = ab & 1;
@@ -177,9 +223,9 @@ __global__ __launch_bounds__(256) void k_theta(const ThetaTask* tasks, int ntask
 hipError_t launch_theta(const ThetaTask* d_tasks, int ntasks, int total_tiles, int fp32, hipStream_t s) {
   if (total_tiles <= 0) return hipSuccess;
   if (fp32)
-    hipLaunchKernelGGL(k_theta<float2>, dim3(total_tiles), dim3(256), 0, s, d_tasks, ntasks);
+    hipLaunchKernelGGL(k_theta_mfma<float2>, dim3(total_tiles), dim3(256), 0, s, d_tasks, ntasks);
   else
-    hipLaunchKernelGGL(k_theta<double2>, dim3(total_tiles), dim3(256), 0, s, d_tasks, ntasks);
+    hipLaunchKernelGGL(k_theta_mfma<double2>, dim3(total_tiles), dim3(256), 0, s, d_tasks, ntasks);
   return hipGetLastError();
 }
 
