@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-circuits", type=int, default=12)
+    ap.add_argument("--lq-min-rank", type=int, default=-1,
+                    help="Jacobi on the double-LQ factor for min(m,n) >= this (0 off; default: engine default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (profiler counter calibration)")
     args = ap.parse_args()
@@ -191,6 +193,8 @@ def main():
         sys.exit(2)
     depth = args.warmup + args.steps
     eng = mps.Engine()
+    if args.lq_min_rank >= 0:
+        eng.set_lq_precondition(args.lq_min_rank)
     eng.set_timing(True)
 
     log(f"batched {args.precision}: {args.batch} x 40-qubit depth-{depth} Haar brickwork, chi_max {CHI}")
@@ -231,6 +235,7 @@ def main():
         "nominal_frac_fp64": round(value * nominal_flops / 1e12 / PEAK_FP64_TFLOPS, 4),
         "fidelity_est": float(np.mean([r["fidelity_est"] for r in res])),
         "roofline": roofline,
+        "jacobi_sweeps_per_svd_chi256": round(prof["sweeps_large"] / max(1, prof["svds_large"]), 2),
     }
     if not args.no_plain:
         log("plain fp64, one circuit at a time (bonds of a layer batched)")

@@ -192,6 +192,12 @@ typedef struct {
 } hipmps_profile_t;
 int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset);
 int hipmps_engine_set_timing(hipmps_engine e, int enabled);
+/* Preconditioning of the SVD: for min(m, n) >= min_rank the one-sided Jacobi runs on R2 of the
+ * double Householder LQ (X = L Q, L^H = L_Z Q_Z, R2 = L_Z^H) instead of on X, which it
+ * converges on in fewer sweeps; 0 turns it off.  Default 64.  No counterpart in the reference,
+ * whose SVD is ExaTN's (ExaTnMpsVisitor.cpp:1623); the singular values and the truncation are
+ * the same either way. */
+int hipmps_engine_set_lq_precondition(hipmps_engine e, int min_rank);
 
 #ifdef __cplusplus
 }

@@ -217,3 +217,25 @@ def test_serving_alone_equals_mixed(eng):
             assert abs(ez[q] - ref.expval_z_parity([q])) < TOL64
         # the same circuit alone and in the mix gives the same numbers
         assert np.abs(ez - a["expval_z"]).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,depth,chi", [(12, 14, 32), (14, 12, 64)])
+def test_lq_preconditioning_same_results(n, depth, chi):
+    """The Jacobi on R2 of the double Householder LQ gives the same truncated MPS as the
+    Jacobi on X itself, and both match the oracle MPS."""
+    say("lq precondition", n, depth, chi)
+    gates = Cc.haar_brickwork(n, depth, seed=31 + n)
+    m = O.Mps(n, cutoff=1e-10, max_bond=chi).run(gates)
+    res = {}
+    for rank in (0, 16):
+        e = mps.Engine()
+        e.set_lq_precondition(rank)
+        s = mps.MpsState(e, n, cutoff=1e-10, max_bond=chi).run(gates)
+        res[rank] = (list(s.bonds()), s.expval_z_all(), s.stats()["discarded_weight"])
+        s.close()
+        e.close()
+    assert res[0][0] == res[16][0] == list(m.bonds())
+    assert np.abs(res[0][1] - res[16][1]).max() < TOL64
+    for q in range(n):
+        assert abs(res[16][1][q] - m.expval_z_parity([q])) < TOL64
+    assert abs(res[16][2] - m.discarded()) < 1e-10

@@ -32,7 +32,7 @@ EXPORTS = [
     "hipmps_stats", "hipmps_norm", "hipmps_expval_z_parity", "hipmps_expval_z_all", "hipmps_expval_zz",
     "hipmps_amplitudes", "hipmps_wavefunction_slice", "hipmps_state_vector", "hipmps_get_site",
     "hipmps_set_site", "hipmps_run_batch", "hipmps_serve_submit", "hipmps_serve_step",
-    "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing",
+    "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing", "hipmps_engine_set_lq_precondition",
     "hipmps_expval_diag", "hipmps_state_clone",
 ]
 
@@ -132,6 +132,7 @@ def lib():
         "hipmps_serve_pending": ([vp, pint, pint], ip),
         "hipmps_engine_profile": ([vp, ctypes.POINTER(Profile), ip], ip),
         "hipmps_engine_set_timing": ([vp, ip], ip),
+        "hipmps_engine_set_lq_precondition": ([vp, ip], ip),
         "hipmps_expval_diag": ([vp, dp, dp], ip),
         "hipmps_state_clone": ([vp, ctypes.POINTER(vp)], ip),
     }

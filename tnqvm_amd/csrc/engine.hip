@@ -158,6 +158,7 @@ struct hipmps_engine_s {
   double tol = 1e-13;       // FP64 path: relative orthogonality of converged rows
   double tol_mixed = 1e-8;  // mixed path: factors are stored in complex64
   int inner_sweeps = 1;
+  int lq_min_rank = 64;  // Jacobi on the double-LQ factor when min(m, n) >= this (0: never)
   DevBuf ws;     // per-wave arena
   DevBuf desc;   // task descriptors
   DevBuf small;  // counters / K / stats
@@ -319,6 +320,8 @@ struct WaveItem {
   void* old_i;
   void* old_j;
   size_t off_theta, off_y, off_vec;
+  int lq;                            // Jacobi on the double-LQ factor R2 of X
+  size_t off_z, off_j, off_yt, off_lq;
 };
 
 int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
@@ -339,6 +342,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     int maxb = s->t.max_bond_dim > 0 ? s->t.max_bond_dim : INT_MAX - 1;
     it.kub = std::min(it.r, maxb);
     it.fp32 = s->fp32;
+    it.lq = E->lq_min_rank > 0 && it.r >= E->lq_min_rank;
   }
   std::stable_sort(items.begin(), items.end(), [](const WaveItem& a, const WaveItem& b) {
     if (a.fp32 != b.fp32) return a.fp32 < b.fp32;
@@ -361,6 +365,17 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     off = align_up(off + (size_t)it.r_pad * it.c * 16);
     it.off_vec = off;
     off = align_up(off + (size_t)it.r_pad * (8 * 5 + 4));
+    if (it.lq) {
+      const size_t panels = (it.r + 15) / 16;  // LQ panels of 16 rows, T factor 16 x 16 each
+      it.off_z = off;  // L^H, then the second LQ (r x r)
+      off = align_up(off + (size_t)it.r * it.r * 16);
+      it.off_j = off;  // R2, the Jacobi matrix (r_pad x r)
+      off = align_up(off + (size_t)it.r_pad * it.r * 16);
+      it.off_yt = off;  // [Y_J | 0] Q~ (r_pad x c)
+      off = align_up(off + (size_t)it.r_pad * it.c * 16);
+      it.off_lq = off;  // W0 and T of both factorizations
+      off = align_up(off + 2 * ((size_t)it.r + panels * 256) * 16);
+    }
   }
   RC_TRY(E->ws.ensure(off));
   char* ws = (char*)E->ws.p;
@@ -372,7 +387,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   double* d_w = (double*)((char*)E->small.p + align_up(3 * N * sizeof(int)));
   // descriptors
   size_t desc_bytes = align_up(N * sizeof(ThetaTask)) + align_up(N * sizeof(SplitTask)) +
-                      align_up(N * sizeof(JacTask)) + align_up(N * sizeof(GemmTask));
+                      align_up(N * sizeof(JacTask)) + align_up(N * sizeof(GemmTask)) +
+                      align_up(2 * N * sizeof(LqTask)) + align_up(3 * N * sizeof(TriTask));
   RC_TRY(E->desc.ensure(desc_bytes));
   // host staging: descriptors | counts (2N ints) | active Jacobi list / results
   const size_t stage_bytes = desc_bytes + align_up(2 * N * sizeof(int)) + N * sizeof(JacTask) + 4 * N * sizeof(double) + 4096;
@@ -382,11 +398,15 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   SplitTask* hsp = (SplitTask*)(hd + align_up(N * sizeof(ThetaTask)));
   JacTask* hjt = (JacTask*)((char*)hsp + align_up(N * sizeof(SplitTask)));
   GemmTask* hgm = (GemmTask*)((char*)hjt + align_up(N * sizeof(JacTask)));
+  LqTask* hlq = (LqTask*)((char*)hgm + align_up(N * sizeof(GemmTask)));
+  TriTask* htr = (TriTask*)((char*)hlq + align_up(2 * N * sizeof(LqTask)));
   char* dd = (char*)E->desc.p;
   ThetaTask* dth = (ThetaTask*)dd;
   SplitTask* dsp = (SplitTask*)(dd + align_up(N * sizeof(ThetaTask)));
   JacTask* djt = (JacTask*)((char*)dsp + align_up(N * sizeof(SplitTask)));
   GemmTask* dgm = (GemmTask*)((char*)djt + align_up(N * sizeof(JacTask)));
+  LqTask* dlq = (LqTask*)((char*)dgm + align_up(N * sizeof(GemmTask)));
+  TriTask* dtr = (TriTask*)((char*)dlq + align_up(2 * N * sizeof(LqTask)));
 
   // group boundaries
   int n64 = 0;
@@ -413,7 +433,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     SplitTask& S = hsp[q];
     memset(&S, 0, sizeof(S));
     S.theta = T.theta;
-    S.Y = (double2*)(ws + it.off_y);
+    S.Y = (double2*)(ws + (it.lq ? it.off_yt : it.off_y));
     S.m = it.m;
     S.n = it.n;
     S.r = it.r;
@@ -442,7 +462,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     S.site_fp32 = it.fp32;
 
     JacTask& J = hjt[q];
-    J.Y = (double2*)S.Y;
+    J.Y = (double2*)(ws + it.off_y);  // X; for LQ items replaced by R2 after the factorizations
     J.ldy = S.ldy;
     J.r = it.r;
     J.r_pad = it.r_pad;
@@ -457,6 +477,34 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     max_rpad = std::max(max_rpad, it.r_pad);
     max_c = std::max(max_c, it.c);
     max_k = std::max(max_k, it.kub);
+  }
+  // double-LQ preconditioning: X = L Q~, L^H = L_Z Q_Z, Jacobi on R2 = L_Z^H
+  int nlq = 0, lq_max_r = 0, lq_max_rpad = 0, lq_max_c = 0;
+  for (int q = 0; q < N; ++q) {
+    auto& it = items[q];
+    if (!it.lq) continue;
+    const size_t panels = (it.r + 15) / 16;
+    double2* Y = (double2*)(ws + it.off_y);
+    double2* Z = (double2*)(ws + it.off_z);
+    double2* Jm = (double2*)(ws + it.off_j);
+    double2* Yt = (double2*)(ws + it.off_yt);
+    double2* lqb = (double2*)(ws + it.off_lq);
+    LqTask& A = hlq[nlq];
+    memset(&A, 0, sizeof(A));
+    A.M = Y; A.ld = it.c; A.rows = it.r; A.cols = it.c;
+    A.W0 = lqb; A.T = lqb + it.r;
+    A.Tgt = Yt; A.ld_t = it.c; A.rows_t = it.r_pad;
+    LqTask& B = hlq[N + nlq];
+    memset(&B, 0, sizeof(B));
+    B.M = Z; B.ld = it.r; B.rows = it.r; B.cols = it.r;
+    B.W0 = lqb + it.r + panels * 256; B.T = B.W0 + it.r;
+    htr[nlq] = TriTask{Y, Z, it.c, it.r, it.r, it.r};               // Z = L^H
+    htr[N + nlq] = TriTask{Z, Jm, it.r, it.r, it.r_pad, it.r};      // J = L_Z^H, pad rows zero
+    htr[2 * N + nlq] = TriTask{Jm, Yt, it.r, it.r, it.r_pad, it.c};  // Yt = [J | 0]
+    ++nlq;
+    lq_max_r = std::max(lq_max_r, it.r);
+    lq_max_rpad = std::max(lq_max_rpad, it.r_pad);
+    lq_max_c = std::max(lq_max_c, it.c);
   }
   // GEMM recovering the second factor, one per item; tile_begin relative to its
   // (precision, transposed) range
@@ -505,6 +553,21 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   if (n64 > 0) HIP_TRY(launch_theta(dth, n64, tile_acc[0], 0, st));
   if (N - n64 > 0) HIP_TRY(launch_theta(dth + n64, N - n64, tile_acc[1], 1, st));
   HIP_TRY(launch_jacobi_prep(dsp, djt, N, max_rpad, max_c, st));
+  if (nlq > 0) {
+    HIP_TRY(launch_lq(dlq, nlq, lq_max_r, st));
+    HIP_TRY(launch_lower_to_upper_h(dtr, nlq, lq_max_r, lq_max_r, st));
+    HIP_TRY(launch_lq(dlq + N, nlq, lq_max_r, st));
+    HIP_TRY(launch_lower_to_upper_h(dtr + N, nlq, lq_max_r, lq_max_rpad, st));
+    for (int q = 0; q < N; ++q) {
+      auto& it = items[q];
+      if (!it.lq) continue;
+      JacTask& J = hjt[q];
+      J.Y = (double2*)(ws + it.off_j);
+      J.ldy = it.r;
+      J.c = it.r;
+      J.c_tot = it.r;
+    }
+  }
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[1], st));
 
   // ---- Jacobi sweeps ----
@@ -545,7 +608,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       long long rot = hcounts[2 * q], vis = hcounts[2 * q + 1];
       E->prof.jacobi_pairs_rotated += rot;
       E->prof.jacobi_pairs_visited += vis;
-      const double c = items[q].c;
+      const double c = items[q].lq ? items[q].r : items[q].c;  // columns the Jacobi works on
       E->prof.jacobi_flops += 8.0 * JP * JP * c * (double)(vis + rot);
       if (rot == 0) sweeps_used[q] = sweep + 1;
       else next.push_back(q);
@@ -553,6 +616,10 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     active.swap(next);
   }
   if (!active.empty()) failed_conv = true;
+  if (nlq > 0) {  // rows of the Jacobi result back to the column space of X
+    HIP_TRY(launch_widen(dtr + 2 * N, nlq, lq_max_c, lq_max_rpad, st));
+    HIP_TRY(launch_lq_apply_q(dlq, nlq, lq_max_r, lq_max_rpad, st));
+  }
   // restore the full JacTask array (not needed further, kept for clarity of ownership)
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[2], st));
 
@@ -1165,6 +1232,13 @@ int hipmps_engine_destroy(hipmps_engine e) {
     if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(e->stream);
   delete e;
+  return 0;
+}
+
+int hipmps_engine_set_lq_precondition(hipmps_engine e, int min_rank) {
+  if (!e) return fail(HIPMPS_E_ARG, "null engine");
+  if (min_rank < 0) return fail(HIPMPS_E_ARG, "min_rank must be >= 0");
+  e->lq_min_rank = min_rank;
   return 0;
 }
 

@@ -91,6 +91,28 @@ struct GemmTask {
   double kd0, kd1;        // diagonal operator on the physical index (Z: 1, -1)
 };
 
+// Row-oriented blocked Householder LQ of M (rows x cols, rows <= cols, ld), 16-row panels.
+// After the factorization row k of M holds L[k][0..k] and, above the diagonal, the tail of
+// the unit reflector row w_k (support columns k..cols-1; head w_k[k] in W0[k]); T holds per
+// panel the 16 x 16 upper-triangular factor with H_k0 ... H_k0+15 = I - W^H T W.
+// Tgt (rows_t x cols, ld_t) is the matrix the inverse transform is applied to.
+struct LqTask {
+  double2* M;
+  double2* W0;
+  double2* T;
+  double2* Tgt;
+  int ld, rows, cols;
+  int ld_t, rows_t;
+  int pad[3];
+};
+
+// Copy tasks around the LQ: src (ld_src), dst (ld_dst), n columns, rows_dst rows of dst.
+struct TriTask {
+  const double2* src;
+  double2* dst;
+  int ld_src, n, rows_dst, ld_dst;
+};
+
 // Chain step for amplitudes / slices: rows of partial contractions v[row][l].
 struct ChainTask {
   const double2* vin;   // rows_in x cl
@@ -112,6 +134,16 @@ hipError_t launch_jacobi_prep(const SplitTask* d_split, const JacTask* d_jac, in
                               int max_rows, int max_cols, hipStream_t s);
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
                                int inner_sweeps, hipStream_t s);
+// Householder LQ of every task's M (panel by panel, one launch per panel).
+hipError_t launch_lq(const LqTask* d_tasks, int ntasks, int max_rows, hipStream_t s);
+// Tgt <- Tgt Q~ (Q~ = H_{rows-1} ... H_0 restricted to the first rows of M's row space, applied
+// panel by panel from the last), i.e. [Y | 0] Q~ for Tgt = [Y | 0].
+hipError_t launch_lq_apply_q(const LqTask* d_tasks, int ntasks, int max_rows, int max_rows_t, hipStream_t s);
+// dst (rows_dst x n, ld ld_dst) = conjugate transpose of the lower triangle of src (n x n, ld
+// ld_src); rows n .. rows_dst - 1 zero.
+hipError_t launch_lower_to_upper_h(const TriTask* d_tasks, int ntasks, int max_n, int max_rows_dst, hipStream_t s);
+// dst[i][j] = src[i][j] for j < n, 0 for n <= j < ld_dst, i < rows_dst.
+hipError_t launch_widen(const TriTask* d_tasks, int ntasks, int max_cols, int max_rows_dst, hipStream_t s);
 hipError_t launch_rownorm(const SplitTask* d_split, int ntasks, int max_r, hipStream_t s);
 hipError_t launch_select(const SplitTask* d_split, int ntasks, hipStream_t s);
 hipError_t launch_gather(const SplitTask* d_split, int ntasks, int max_k, int max_len, int fp32,

@@ -90,6 +90,10 @@ class Engine:
     def set_timing(self, on: bool):
         check(lib().hipmps_engine_set_timing(self.h, int(on)))
 
+    def set_lq_precondition(self, min_rank: int):
+        """Jacobi on the double-LQ factor for min(m, n) >= min_rank (0: never)."""
+        check(lib().hipmps_engine_set_lq_precondition(self.h, int(min_rank)))
+
     def profile(self, reset: bool = False) -> dict:
         p = C.Profile()
         check(lib().hipmps_engine_profile(self.h, ctypes.byref(p), int(reset)))
