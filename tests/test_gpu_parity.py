@@ -239,3 +239,36 @@ def test_lq_preconditioning_same_results(n, depth, chi):
     for q in range(n):
         assert abs(res[16][1][q] - m.expval_z_parity([q])) < TOL64
     assert abs(res[16][2] - m.discarded()) < 1e-10
+
+
+def _graded(m, n, seed, decades=12.0):
+    """m x n matrix with known singular values logspace(0, -decades, min(m, n))."""
+    rng = np.random.default_rng(seed)
+    k = min(m, n)
+    u = np.linalg.qr(rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k)))[0]
+    v = np.linalg.qr(rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k)))[0]
+    s = np.logspace(0.0, -decades, k)
+    return (u * s) @ v.conj().T, s
+
+
+@pytest.mark.parametrize("m,n", [(1024, 1024), (512, 1024), (1024, 512)])
+def test_svd_large_graded(eng, m, n):
+    """Bond dimension 512: theta' up to 1024 x 1024 through the LQ-preconditioned Jacobi, with
+    singular values spread over 12 decades (known exactly by construction)."""
+    say("svd large", m, n)
+    a, sref = _graded(m, n, seed=m + 3 * n)
+    cl, cr = m // 2, n // 2
+    s = mps.MpsState(eng, 4)
+    s.set_site(0, np.ones((1, 2, cl)) / np.sqrt(2 * cl))
+    s.set_site(1, a.reshape(cl, 2, n))
+    s.set_site(2, np.eye(n).reshape(n, 2, cr))
+    s.set_site(3, np.ones((cr, 2, 1)) / np.sqrt(2 * cr))
+    s.apply({"name": "unitary", "qubits": (1, 2), "matrix": np.eye(4)})
+    s.flush()
+    A1 = s.get_site(1).reshape(m, -1)
+    A2 = s.get_site(2).reshape(-1, n)
+    sig = (np.abs(A1) ** 2).sum(0)
+    assert len(sig) == min(m, n)
+    assert np.abs(sig - sref).max() < 1e-12
+    assert np.abs(A1 @ A2 - a).max() < 1e-12
+    s.close()
