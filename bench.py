@@ -12,7 +12,14 @@ the interior bonds (32 of the 39 bonds) and are timed.
 Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
   plain_fp64_value   the same layers, one circuit at a time, complex128 everywhere
   speedup_vs_plain   value / plain_fp64_value
-  latency_ms_p50/p99 circuit latency in a continuous-batching serving run
+  uneven_batch       256 brickwork circuits of uneven size (qubits U{16..48}, depth
+                     U{10..60}, chi_max from {32,64,128,256}) submitted together, against
+                     each circuit run alone in plain FP64; circuits/s, applies/s, the ratio,
+                     and the <Z> difference between the two paths per circuit
+  serving            a Poisson arrival stream of such circuits and a 6x6 grid fSim circuit
+                     (chi_max 1024) through the continuous-batching loop and through the
+                     plain path one at a time: circuits/s and p50/p99 latency for both
+  latency_ms_p50/p99 the batched serving latencies (arrival to result)
   roofline           k_jacobi_round (the dominant kernel), FP64 MFMA bound
   cpu_baseline       the FP64 CPU oracle on the host cores at chi = 256
 """
@@ -75,29 +82,101 @@ def run_layers(eng, circuits, precision, warmup, steps):
     return applies, total, step_ms, prof, results
 
 
-def serving_latency(eng, n_circuits, depth, rate_per_step, precision, seed):
-    """Poisson arrivals (in units of serving steps); returns per-circuit latency in ms."""
+def uneven_batch(count, seed):
+    """Brickwork circuits of uneven size: qubits ~ U{16..48}, depth ~ U{10..60}, chi_max drawn
+    from {32, 64, 128, 256}, Haar 2x2 and 4x4 gates, svd-cutoff 1e-10."""
+    return [{"n": c["n"], "gates": c["gates"], "chi": c["chi"], "kind": "brickwork"}
+            for c in C.mixed_batch(count, seed=seed)]
+
+
+def serving_stream(brick, n_grid, grid_cycles, seed):
+    """The serving workload: the given brickwork circuits mixed with 6x6 grid circuits
+    (fSim(pi/2, pi/6) on the ABCDCDAB coupler patterns, sqrt(X)/sqrt(Y)/sqrt(W) single-qubit
+    gates, snake map, routed to nearest neighbours) at chi_max 1024, in a seeded random order.
+    Each job keeps its index in `brick` ("src", -1 for grid circuits)."""
     rng = np.random.default_rng(seed)
+    jobs = [dict(j, src=k) for k, j in enumerate(brick)]
+    for k in range(n_grid):
+        g = C.route_nearest_neighbour(C.grid_fsim(6, 6, grid_cycles, seed=seed + 2 + k))
+        jobs.append({"n": 36, "gates": g, "chi": 1024, "kind": "grid", "src": -1})
+    return [jobs[i] for i in rng.permutation(len(jobs))]
+
+
+def fifo_latency(arrivals, service):
+    """Latency (ms) and span (s) of a single server that runs jobs one at a time in arrival
+    order with the given service times (s): finish_k = max(arrival_k, finish_(k-1)) + service_k."""
+    t, lat = 0.0, np.zeros(len(arrivals))
+    for k, (a, s) in enumerate(zip(arrivals, service)):
+        t = max(t, a) + s
+        lat[k] = 1e3 * (t - a)
+    return lat, t - arrivals[0]
+
+
+def serve_batched(eng, jobs, arrivals, precision):
+    """Continuous batching: every circuit whose arrival time (s, wall clock) has passed is
+    submitted at the next step boundary and advances with all others in flight. Returns
+    per-circuit latency (arrival -> result observed, ms), the span from the first arrival to
+    the last result (s), and the results."""
     srv = mps.Server(eng)
-    tr = mps.trunc_opts(cutoff=CUTOFF, max_bond=CHI, precision=precision)
-    pending = [C.haar_brickwork(N_QUBITS, depth, seed=0x7E5A1000 + k) for k in range(n_circuits)]
-    arrivals = np.cumsum(rng.exponential(1.0 / rate_per_step, size=n_circuits))
-    live, lat, step = [], [], 0
-    nxt = 0
+    n = len(jobs)
+    lat, res = np.zeros(n), [None] * n
+    live, nxt, done = {}, 0, 0
     t0 = time.perf_counter()
-    while len(lat) < n_circuits:
-        while nxt < n_circuits and arrivals[nxt] <= step:
-            live.append(srv.submit(N_QUBITS, pending[nxt], tr))
+    while done < n:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            j = jobs[nxt]
+            tk = srv.submit(j["n"], j["gates"], mps.trunc_opts(cutoff=CUTOFF, max_bond=j["chi"], precision=precision))
+            live[tk] = nxt
             nxt += 1
+        if not live:
+            time.sleep(max(0.0, arrivals[nxt] - now))
+            continue
         srv.step()
-        step += 1
-        for t in list(live):
-            r = srv.poll(t)
+        now = time.perf_counter() - t0
+        for tk in list(live):
+            r = srv.poll(tk)
             if r is not None:
-                lat.append(r["latency_ms"])
-                live.remove(t)
-    wall = time.perf_counter() - t0
-    return np.array(lat), wall
+                k = live.pop(tk)
+                lat[k] = 1e3 * (now - arrivals[k])
+                res[k] = r
+                done += 1
+    return lat, time.perf_counter() - t0 - arrivals[0], res
+
+
+def serve_plain(eng, jobs, arrivals):
+    """The plain per-circuit FP64 path on the same arrivals: one circuit at a time in arrival
+    order (complex128 everywhere, the bonds of one layer batched), <Z> of every qubit read
+    back before the next circuit starts. Returns latency (ms), span (s), <Z> per circuit and
+    each circuit's own run time (s)."""
+    n = len(jobs)
+    lat, ez, own = np.zeros(n), [None] * n, np.zeros(n)
+    t0 = time.perf_counter()
+    for k, j in enumerate(jobs):
+        wait = arrivals[k] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ts = time.perf_counter()
+        s = mps.MpsState(eng, j["n"], cutoff=CUTOFF, max_bond=j["chi"], precision="fp64")
+        ez[k] = s.run(j["gates"]).expval_z_all()
+        s.close()
+        own[k] = time.perf_counter() - ts
+        lat[k] = 1e3 * (time.perf_counter() - t0 - arrivals[k])
+    return lat, time.perf_counter() - t0 - arrivals[0], ez, own
+
+
+def fp64_sensitivity(eng, job, ez_ref):
+    """How far the plain FP64 <Z> moves when only the gate matrices are rounded to complex64
+    (a relative perturbation of about 6e-8, the size of one complex64 rounding): the
+    conditioning of the truncated result itself, for comparison with the mixed path's error."""
+    gates = []
+    for g in job["gates"]:
+        m = mps.matrix_of(g).astype(np.complex64).astype(np.complex128)
+        gates.append({"name": "unitary", "qubits": g["qubits"], "matrix": m})
+    s = mps.MpsState(eng, job["n"], cutoff=CUTOFF, max_bond=job["chi"], precision="fp64")
+    ez = s.run(gates).expval_z_all()
+    s.close()
+    return float(np.abs(ez - ez_ref).max())
 
 
 def gate_at_a_time(eng, circuit, warmup, steps, max_applies=40):
@@ -177,7 +256,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-circuits", type=int, default=12)
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--uneven-circuits", type=int, default=256, help="circuits of the uneven batch")
+    ap.add_argument("--serve-circuits", type=int, default=96, help="brickwork circuits in the serving stream")
+    ap.add_argument("--serve-grid", type=int, default=1, help="6x6 grid circuits in the serving stream")
+    ap.add_argument("--serve-grid-cycles", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, circuits/s")
     ap.add_argument("--lq-min-rank", type=int, default=-1,
                     help="Jacobi on the double-LQ factor for min(m,n) >= this (0 off; default: engine default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -248,13 +332,77 @@ def main():
         out["plain_fp64_gate_at_a_time_value"] = round(ga / gt, 2)
         out["speedup_vs_gate_at_a_time"] = round(value / (ga / gt), 3)
         log(f"  {ga} applies in {gt:.3f} s -> {ga / gt:.1f} applies/s")
+    plain_own = None  # plain FP64 run time (s) of each uneven-batch circuit
+    uneven = uneven_batch(args.uneven_circuits, 0x7E5A0004)
+    if not args.no_uneven:
+        jobs = uneven
+        applies_u = sum(C.count_2q(j["gates"]) for j in jobs)
+        log(f"uneven batch: {len(jobs)} brickwork circuits ({applies_u} 2q gates) submitted together, "
+            f"{args.precision}")
+        _, bspan, bres = serve_batched(eng, jobs, np.zeros(len(jobs)), args.precision)
+        log(f"  {len(jobs) / bspan:.2f} circuits/s, {applies_u / bspan:.1f} applies/s")
+        ub = {"workload": "Haar brickwork, qubits U{16..48}, depth U{10..60}, chi_max from {32,64,128,256}, "
+                          "svd-cutoff 1e-10, all submitted at once",
+              "circuits": len(jobs), "applies": applies_u, "precision": args.precision, "span_s": round(bspan, 3),
+              "circuits_per_s": round(len(jobs) / bspan, 3), "applies_per_s": round(applies_u / bspan, 2)}
+        if not args.no_plain:
+            log("  plain fp64, each circuit alone")
+            _, _, pez, plain_own = serve_plain(eng, jobs, np.zeros(len(jobs)))
+            ptot = float(plain_own.sum())
+            log(f"  plain {len(jobs) / ptot:.2f} circuits/s -> x{ptot / bspan:.2f}")
+            dz = np.array([np.abs(r["expval_z"] - z).max() for r, z in zip(bres, pez)])
+            fid = np.array([r["fidelity_est"] for r in bres])
+            worst = int(np.argmax(dz))
+            ub["plain_fp64"] = {"span_s": round(ptot, 3), "circuits_per_s": round(len(jobs) / ptot, 3),
+                                "applies_per_s": round(applies_u / ptot, 2)}
+            ub["speedup_vs_plain"] = round(ptot / bspan, 3)
+            ub["z_error_vs_plain_fp64"] = {
+                "max": float(dz.max()), "median": float(np.median(dz)),
+                "circuits_within_1e-5": int((dz <= 1e-5).sum()),
+                "worst_circuit": {"qubits": jobs[worst]["n"], "chi_max": jobs[worst]["chi"],
+                                  "fidelity_est": float(fid[worst]),
+                                  "fp64_sensitivity": fp64_sensitivity(eng, jobs[worst], pez[worst])}}
+        out["uneven_batch"] = ub
     if not args.no_serving:
-        log(f"serving: {args.serve_circuits} circuits (depth 20), Poisson arrivals")
-        lat, wall = serving_latency(eng, args.serve_circuits, 20, 0.5, args.precision, 0x5E7)
+        jobs = serving_stream(uneven[:args.serve_circuits], args.serve_grid, args.serve_grid_cycles, 0x7E5A0010)
+        arrivals = np.cumsum(np.random.default_rng(0x5E7).exponential(1.0 / args.serve_rate, size=len(jobs)))
+        arrivals -= arrivals[0]
+        log(f"serving: {args.serve_circuits} brickwork + {args.serve_grid} 6x6 grid circuits, Poisson arrivals "
+            f"at {args.serve_rate}/s")
+        lat, span, _ = serve_batched(eng, jobs, arrivals, args.precision)
+        grid_lat = [float(lat[k]) for k, j in enumerate(jobs) if j["kind"] == "grid"]
+        log(f"  batched {args.precision}: {len(jobs) / span:.2f} circuits/s, p50 {np.percentile(lat, 50):.0f} ms, "
+            f"p99 {np.percentile(lat, 99):.0f} ms, grid {grid_lat}")
         out["latency_ms_p50"] = round(float(np.percentile(lat, 50)), 2)
         out["latency_ms_p99"] = round(float(np.percentile(lat, 99)), 2)
-        out["serving"] = {"circuits": args.serve_circuits, "depth": 20, "circuits_per_s": round(len(lat) / wall, 3),
-                          "arrival_rate_per_step": 0.5}
+        serving = {"workload": f"Poisson arrivals of {args.serve_circuits} of the uneven brickwork circuits and "
+                               f"{args.serve_grid} 6x6 grid fSim circuit(s) ({args.serve_grid_cycles} cycles, snake "
+                               "map, routed, chi_max 1024), svd-cutoff 1e-10",
+                   "circuits": len(jobs), "brickwork": args.serve_circuits, "grid_6x6": args.serve_grid,
+                   "arrival_rate_per_s": args.serve_rate, "precision": args.precision,
+                   "circuits_per_s": round(len(jobs) / span, 3),
+                   "latency_ms_p50": out["latency_ms_p50"], "latency_ms_p99": out["latency_ms_p99"],
+                   "grid_latency_ms": [round(x, 1) for x in grid_lat]}
+        if not args.no_plain:
+            # the plain path serves one circuit at a time, so its latencies follow from each
+            # circuit's measured plain run time through the FIFO recurrence
+            need = [k for k, j in enumerate(jobs) if j["src"] < 0 or plain_own is None]
+            _, _, _, own_new = serve_plain(eng, [jobs[k] for k in need], np.zeros(len(need)))
+            service = np.zeros(len(jobs))
+            for k, j in enumerate(jobs):
+                if j["src"] >= 0 and plain_own is not None:
+                    service[k] = plain_own[j["src"]]
+            service[need] = own_new
+            plat, pspan = fifo_latency(arrivals, service)
+            log(f"  plain fp64: {len(jobs) / pspan:.2f} circuits/s, p50 {np.percentile(plat, 50):.0f} ms, "
+                f"p99 {np.percentile(plat, 99):.0f} ms")
+            serving["plain_fp64"] = {"circuits_per_s": round(len(jobs) / pspan, 3),
+                                     "latency_ms_p50": round(float(np.percentile(plat, 50)), 2),
+                                     "latency_ms_p99": round(float(np.percentile(plat, 99)), 2),
+                                     "method": "FIFO recurrence over the measured plain FP64 run time of each "
+                                               "circuit (one circuit at a time)"}
+            serving["speedup_vs_plain"] = round(pspan / span, 3)
+        out["serving"] = serving
     eng.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")

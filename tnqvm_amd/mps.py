@@ -36,20 +36,28 @@ def matrix_of(g) -> np.ndarray:
     return gate_matrix(g["name"], g.get("params", ()))
 
 
+_GATE_DTYPE = np.dtype([("nq", np.int32), ("q0", np.int32), ("q1", np.int32), ("pad", np.int32),
+                        ("m", np.float64, 32)])  # the layout of hipmps_gate
+
+
 def to_c_gates(gates):
     """Gate dicts -> ctypes array of hipmps_gate (Measure / I are skipped)."""
     gl = [g for g in gates if g["name"] not in ("Measure", "I")]
-    arr = (C.Gate * max(1, len(gl)))()
-    for k, g in enumerate(gl):
-        m = matrix_of(g).reshape(-1)
-        q = g["qubits"]
-        arr[k].nq = len(q)
-        arr[k].q0 = int(q[0])
-        arr[k].q1 = int(q[1]) if len(q) > 1 else -1
-        flat = np.zeros(32)
-        flat[: 2 * m.size] = m.view(np.float64)
-        arr[k].m[:] = flat.tolist()
-    return arr, len(gl)
+    n = len(gl)
+    buf = np.zeros(max(1, n), dtype=_GATE_DTYPE)
+    if n:
+        mats = [matrix_of(g).reshape(-1) for g in gl]
+        nq = np.fromiter((len(g["qubits"]) for g in gl), np.int32, n)
+        buf["nq"][:n] = nq
+        buf["q0"][:n] = np.fromiter((g["qubits"][0] for g in gl), np.int32, n)
+        buf["q1"][:n] = np.fromiter((g["qubits"][1] if len(g["qubits"]) > 1 else -1 for g in gl), np.int32, n)
+        for width, nqv in ((4, 1), (16, 2)):
+            idx = np.flatnonzero(nq == nqv)
+            if len(idx):
+                buf["m"][idx, : 2 * width] = np.stack([mats[k] for k in idx]).view(np.float64)
+    assert _GATE_DTYPE.itemsize == ctypes.sizeof(C.Gate)
+    arr = (C.Gate * len(buf)).from_buffer(buf)
+    return arr, n
 
 
 def trunc_opts(cutoff=DBL_MIN, max_bond=0, precision="fp64", renormalize=False, partial_norm="squared",
