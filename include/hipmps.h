@@ -105,6 +105,9 @@ typedef struct {
   int jacobi_sweeps_max;   /* worst sweep count of one SVD */
   long long jacobi_sweeps_total;
   size_t device_bytes;     /* device memory held by the state */
+  int norm_warnings;       /* SVDs after which a sqrt(S) factor had Frobenius norm < 1e-3: the
+                              post-SVD validation of ExaTnMpsVisitor.cpp:1634-1661, which
+                              prints an error there and continues (its assert is debug-only) */
 } hipmps_stats_t;
 
 /* ---- engine ---------------------------------------------------------------------- */

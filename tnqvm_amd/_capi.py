@@ -23,6 +23,8 @@ ERRORS = {
     -1: "HIPMPS_E_ARG", -2: "HIPMPS_E_NONADJACENT", -3: "HIPMPS_E_NOMEM", -4: "HIPMPS_E_CONVERGENCE",
     -5: "HIPMPS_E_DEVICE", -6: "HIPMPS_E_NUMERIC", -7: "HIPMPS_E_UNSUPPORTED",
 }
+HIPMPS_E_ARG, HIPMPS_E_NONADJACENT, HIPMPS_E_NOMEM, HIPMPS_E_CONVERGENCE = -1, -2, -3, -4
+HIPMPS_E_DEVICE, HIPMPS_E_NUMERIC, HIPMPS_E_UNSUPPORTED = -5, -6, -7
 
 # every symbol declared in include/hipmps.h
 EXPORTS = [
@@ -63,7 +65,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("norm", ctypes.c_double), ("discarded_weight", ctypes.c_double),
                 ("fidelity_est", ctypes.c_double), ("max_bond", ctypes.c_int), ("n_2q", ctypes.c_int),
                 ("n_1q", ctypes.c_int), ("jacobi_sweeps_max", ctypes.c_int),
-                ("jacobi_sweeps_total", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t)]
+                ("jacobi_sweeps_total", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t),
+                ("norm_warnings", ctypes.c_int)]
 
 
 class Circuit(ctypes.Structure):

@@ -12,6 +12,7 @@ namespace hipmps {
 // Block size of the one-sided block-Jacobi SVD: a block pair is 2*JB rows.
 constexpr int JB = 16;
 constexpr int JP = 2 * JB;
+constexpr int WS_N = 6;  // doubles of split statistics per SVD (SplitTask::wstats)
 
 // theta' = G . (A_i x A_j): merge of two neighbouring sites and the 2-qubit gate
 // (ExaTnMpsVisitor.cpp:1401-1524), written as the (2cl) x (2cr) matrix with
@@ -58,7 +59,8 @@ struct SplitTask {
   double* scale_w;       // K: target norm (f sigma)^a of the recovered factor
   double* inv_sig;       // K: 1 / sigma_k (0 for sigma_k = 0), GEMM operand scale
   int* K;                // out: new bond dimension
-  double* wstats;        // out: [discarded, kept, total, renorm factor]
+  double* wstats;        // out: [discarded, kept, total, renorm factor, sum of all sigma,
+                         //       1 if a singular value was NaN / Inf]
   void* Ai;              // out site i: cl x 2 x K
   void* Aj;              // out site j: K x 2 x cr
   int cl, cr;

@@ -571,11 +571,27 @@ __global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
   int P = 1;
   while (P < r) P <<= 1;
   const int tid = threadIdx.x;
+  __shared__ int s_bad;
+  if (tid == 0) {
+    s_first = r;
+    s_bad = 0;
+  }
+  __syncthreads();
+  // a NaN or Inf singular value (non-finite input or overflow) is sorted as 0 and flagged:
+  // the keys of real rows then stay >= 0 > -1 (the padding), so the first r indices of the
+  // sorted order are always rows of Y
   for (int i = tid; i < P; i += 256) {
-    key[i] = i < r ? S.rownorm[i] : -1.0;
+    double v = -1.0;
+    if (i < r) {
+      v = S.rownorm[i];
+      if (!(v >= 0.0 && v <= DBL_MAX)) {
+        v = 0.0;
+        s_bad = 1;
+      }
+    }
+    key[i] = v;
     idx[i] = i;
   }
-  if (tid == 0) s_first = r;
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
@@ -621,6 +637,15 @@ __global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) { if (tid < w) red[tid] += red[tid + w]; __syncthreads(); }
   disc = red[0];
+  __syncthreads();
+  // sum of all singular values: the squared Frobenius norm of either sqrt(S) factor
+  // before truncation, for the post-SVD norm validation (ExaTnMpsVisitor.cpp:1634-1661)
+  double ssum = 0.0;
+  for (int i = tid; i < r; i += 256) ssum += key[i];
+  red[tid] = ssum;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) { if (tid < w) red[tid] += red[tid + w]; __syncthreads(); }
+  ssum = red[0];
   double f = 1.0;
   if (S.renorm && kept > 0.0) f = sqrt((kept + disc) / kept);
   // scales: left factor U (s)^aL, right factor (s)^aR V^H with s = f * sigma.
@@ -652,6 +677,8 @@ __global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
     S.wstats[1] = kept;
     S.wstats[2] = kept + disc;
     S.wstats[3] = f;
+    S.wstats[4] = ssum;
+    S.wstats[5] = s_bad ? 1.0 : 0.0;
   }
 }
 

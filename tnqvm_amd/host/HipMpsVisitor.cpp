@@ -178,6 +178,10 @@ void HipMpsVisitor::finalize() {
   buffer->addExtraInfo("discarded-weight", st.discarded_weight);
   buffer->addExtraInfo("truncation-fidelity", st.fidelity_est);
   buffer->addExtraInfo("max-bond-dim-reached", st.max_bond);
+  buffer->addExtraInfo("norm-validation-failures", st.norm_warnings);
+  if (st.norm_warnings > 0)
+    std::cerr << "[ERROR] Tensor norm validation failed after " << st.norm_warnings
+              << " SVD(s): a factor's Frobenius norm fell below 1e-3\n";
 
   if (options.keyExists<bool>("exp-val-z-per-qubit") && options.get<bool>("exp-val-z-per-qubit")) {
     std::vector<double> z(m_n);
