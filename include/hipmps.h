@@ -144,9 +144,26 @@ int hipmps_expval_diag(hipmps_state s, const double* diag_2n, double* out);
 /* deep copy of a state on the same engine (observable circuits in VQE mode) */
 int hipmps_state_clone(hipmps_state src, hipmps_state* out);
 int hipmps_state_vector(hipmps_state s, double* out_re_im, size_t cap); /* n <= 26 */
-/* copy one site tensor (chi_l x 2 x chi_r, complex128) to / from the host */
+/* copy one site tensor (chi_l x 2 x kap x chi_r, complex128; kap = 1 unless noise channels
+ * were applied) to / from the host; set_site writes a site with kap = 1 */
 int hipmps_get_site(hipmps_state s, int k, double* out, size_t cap);
 int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* in);
+
+/* ---- noisy simulation: locally purified MPS (exatn-pmps) ------------------------------
+ * A state becomes a purified MPS the first time a channel is applied: every site then has a
+ * Kraus leg kap_k (A[l][p][k][r]) and the state represents rho = sum_k |psi_k><psi_k|.
+ * Gates, <Z>, <ZZ>, Z parities and diagonal operators (hipmps_expval_diag: outcome
+ * probabilities) apply to rho; amplitudes, slices and the state vector are refused
+ * (HIPMPS_E_UNSUPPORTED).
+ *
+ * hipmps_apply_kraus replaces ExaTnPmpsVisitor::applyKrausOp / applyLocalKrausOp
+ * (ExaTnPmpsVisitor.cpp:887-1130): rho -> sum_o K_o rho K_o^H on qubit q, with n_ops 2 x 2
+ * complex Kraus operators (row-major, 8 doubles each, 1 <= n_ops <= HIPMPS_MAX_KRAUS_OPS).
+ * The Kraus leg is compressed as the reference does: vectors with partial norm below 1e-9
+ * dropped, at most min(2 chi_l chi_r, 2 kap) kept.  Gates and channels apply in call order. */
+#define HIPMPS_MAX_KRAUS_OPS 16
+int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats);
+int hipmps_kraus_dims(hipmps_state s, int* out_n); /* kap_k per site */
 
 /* ---- many circuits advanced together (no counterpart in the reference) ------------- */
 typedef struct {

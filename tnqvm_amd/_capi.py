@@ -35,7 +35,7 @@ EXPORTS = [
     "hipmps_amplitudes", "hipmps_wavefunction_slice", "hipmps_state_vector", "hipmps_get_site",
     "hipmps_set_site", "hipmps_run_batch", "hipmps_serve_submit", "hipmps_serve_step",
     "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing", "hipmps_engine_set_lq_precondition",
-    "hipmps_expval_diag", "hipmps_state_clone",
+    "hipmps_expval_diag", "hipmps_state_clone", "hipmps_apply_kraus", "hipmps_kraus_dims",
 ]
 
 
@@ -138,6 +138,8 @@ def lib():
         "hipmps_engine_set_lq_precondition": ([vp, ip], ip),
         "hipmps_expval_diag": ([vp, dp, dp], ip),
         "hipmps_state_clone": ([vp, ctypes.POINTER(vp)], ip),
+        "hipmps_apply_kraus": ([vp, ip, ip, dp], ip),
+        "hipmps_kraus_dims": ([vp, pint], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

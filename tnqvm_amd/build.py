@@ -29,7 +29,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"),
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
-DEVICE_SOURCES = ["kernels.hip", "lq.hip", "engine.hip", "gates.cpp"]
+DEVICE_SOURCES = ["kernels.hip", "lq.hip", "kraus.hip", "engine.hip", "gates.cpp"]
 
 
 def _run(cmd):

@@ -183,6 +183,7 @@ struct hipmps_state_s {
   int fp32 = 0;
   size_t es = 16;
   std::vector<int> chi;
+  std::vector<int> kap;  // Kraus leg of each site (1 everywhere for a pure state)
   std::vector<Site> sites;
   std::vector<std::array<cd, 4>> pend;
   std::vector<char> has_pend;
@@ -201,6 +202,7 @@ struct hipmps_state_s {
   size_t bytes = 0;
   int status = 0;
   int norm_warnings = 0;  // SVDs whose factor norm fell below 1e-3 (post-SVD validation)
+  int kraus_ops = 0;      // noise channels applied
   std::vector<void*> deferred;
 };
 
@@ -235,6 +237,20 @@ size_t projected_bytes(int n, int maxb, int fp32) {
   size_t es = fp32 ? 8 : 16, tot = 0;
   for (int k = 0; k < n; ++k) tot += bond_bound(n, k, maxb) * 2 * bond_bound(n, k + 1, maxb) * es;
   return tot;
+}
+
+size_t site_elems(const hipmps_state_s* s, int k) { return (size_t)s->chi[k] * 2 * s->kap[k] * s->chi[k + 1]; }
+
+bool is_purified(const hipmps_state_s* s) {
+  for (int k : s->kap)
+    if (k > 1) return true;
+  return false;
+}
+
+int max_kap(const hipmps_state_s* s) {
+  int m = 1;
+  for (int k : s->kap) m = std::max(m, k);
+  return m;
 }
 
 int ensure_site(hipmps_state_s* s, int k, size_t elems, bool keep_old_for_read, void** old_out) {
@@ -290,6 +306,7 @@ int upload_site_host(hipmps_state_s* s, int k, const cd* data, size_t n) {
 int state_init_product(hipmps_state_s* s) {
   // |0..0>: every site 1 x 2 x 1 with body {1, 0} (ExaTnMpsVisitor.cpp:46, :323)
   s->chi.assign(s->n + 1, 1);
+  s->kap.assign(s->n, 1);
   const cd body[2] = {1.0, 0.0};
   for (int k = 0; k < s->n; ++k) {
     RC_TRY(ensure_site(s, k, 2, false, nullptr));
@@ -307,6 +324,7 @@ int state_init_product(hipmps_state_s* s) {
   s->sweeps_total = 0;
   s->status = 0;
   s->norm_warnings = 0;
+  s->kraus_ops = 0;
   return 0;
 }
 
@@ -324,6 +342,8 @@ struct WaveItem {
   size_t off_theta, off_y, off_vec;
   int lq;                            // Jacobi on the double-LQ factor R2 of X
   size_t off_z, off_j, off_yt, off_lq;
+  int ki, kj;                        // Kraus legs of the two sites (fused into cl and cr)
+  size_t off_pin, off_pout;          // left site with its Kraus leg moved next to the left bond
 };
 
 int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
@@ -332,9 +352,11 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   // order: fp64 before fp32, non-transposed before transposed (contiguous launch groups)
   for (auto& it : items) {
     hipmps_state_s* s = it.s;
-    it.cl = s->chi[it.i];
+    it.ki = s->kap[it.i];
+    it.kj = s->kap[it.i + 1];
+    it.cl = s->chi[it.i] * it.ki;
     it.cm = s->chi[it.i + 1];
-    it.cr = s->chi[it.i + 2];
+    it.cr = it.kj * s->chi[it.i + 2];
     it.m = 2 * it.cl;
     it.n = 2 * it.cr;
     it.r = std::min(it.m, it.n);
@@ -378,6 +400,13 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       it.off_lq = off;  // W0 and T of both factorizations
       off = align_up(off + 2 * ((size_t)it.r + panels * 256) * 16);
     }
+    if (it.ki > 1) {
+      const size_t es = it.fp32 ? 8 : 16;
+      it.off_pin = off;  // old left site as [l][k][p][m]
+      off = align_up(off + (size_t)it.cl * 2 * it.cm * es);
+      it.off_pout = off;  // new left site as [l][k][p][K]
+      off = align_up(off + (size_t)it.cl * 2 * it.kub * es);
+    }
   }
   RC_TRY(E->ws.ensure(off));
   char* ws = (char*)E->ws.p;
@@ -420,7 +449,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     const int grp = it.fp32;
     ThetaTask& T = hth[q];
     memset(&T, 0, sizeof(T));
-    T.Ai = it.old_i;
+    T.Ai = it.ki > 1 ? (void*)(ws + it.off_pin) : it.old_i;
     T.Aj = it.old_j;
     T.theta = (double2*)(ws + it.off_theta);
     T.cl = it.cl;
@@ -451,7 +480,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     S.perm = (int*)(v + 40 * it.r_pad);
     S.K = d_K + q;
     S.wstats = d_w + WS_N * q;
-    S.Ai = it.s->sites[it.i].p;
+    S.Ai = it.ki > 1 ? (void*)(ws + it.off_pout) : it.s->sites[it.i].p;
     S.Aj = it.s->sites[it.i + 1].p;
     S.cl = it.cl;
     S.cr = it.cr;
@@ -551,6 +580,10 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   HIP_TRY(hipMemcpyAsync(dd, hd, desc_bytes, hipMemcpyHostToDevice, st));
 
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[0], st));
+  // purified left sites: Kraus leg next to the left bond, [l][p][k][m] -> [l][k][p][m]
+  for (auto& it : items)
+    if (it.ki > 1)
+      HIP_TRY(launch_swap_mid(it.old_i, ws + it.off_pin, it.s->chi[it.i], 2, it.ki, it.cm, nullptr, it.fp32, st));
   // ---- theta ----
   if (n64 > 0) HIP_TRY(launch_theta(dth, n64, tile_acc[0], 0, st));
   if (N - n64 > 0) HIP_TRY(launch_theta(dth + n64, N - n64, tile_acc[1], 1, st));
@@ -643,6 +676,13 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     }
     if (n64 > 0) HIP_TRY(launch_normalize(dsp, n64, mk64, 0, st));
     if (N - n64 > 0) HIP_TRY(launch_normalize(dsp + n64, N - n64, mk32, 1, st));
+  }
+  // purified left sites back to [l][p][k][K]
+  for (int q = 0; q < N; ++q) {
+    auto& it = items[q];
+    if (it.ki > 1)
+      HIP_TRY(launch_swap_mid(ws + it.off_pout, it.s->sites[it.i].p, it.s->chi[it.i], it.ki, 2, it.kub, d_K + q,
+                              it.fp32, st));
   }
   // results
   int* hK = hcounts;  // reuse staging
@@ -764,7 +804,7 @@ int flush_pending_1q(hipmps_state_s* s) {
     if (!s->has_pend[q]) continue;
     double2 m[4];
     for (int k = 0; k < 4; ++k) m[k] = make_double2(s->pend[q][k].real(), s->pend[q][k].imag());
-    HIP_TRY(launch_apply_1q(s->sites[q].p, s->fp32, s->chi[q], s->chi[q + 1], m, s->e->stream));
+    HIP_TRY(launch_apply_1q(s->sites[q].p, s->fp32, s->chi[q], s->kap[q] * s->chi[q + 1], m, s->e->stream));
     s->has_pend[q] = 0;
     s->pend[q] = {cd(1), cd(0), cd(0), cd(1)};
   }
@@ -776,6 +816,116 @@ int state_flush(hipmps_state_s* s) {
   RC_TRY(state_run_wave(s));
   RC_TRY(flush_pending_1q(s));
   HIP_TRY(hipStreamSynchronize(s->e->stream));
+  return 0;
+}
+
+// Noise channel rho -> sum_o K_o rho K_o^H on qubit q (ExaTnPmpsVisitor.cpp:887-1130).  The
+// reference merges the site with its conjugate over the Kraus leg, applies the channel's Choi
+// operator and splits the result back; the same site comes out here without forming that
+// (2 chiL chiR)^2 operator: the operators open one row per (old Kraus index, operator), and a
+// one-sided FP64 Jacobi on those rows gives orthogonal Kraus vectors sigma_j v_j (the
+// reference's U sqrt(S) of the merged operator, up to a unitary on the Kraus leg, which
+// leaves rho unchanged).  Kept: the rows up to the first whose partial norm is below 1e-9 (that
+// row excluded, ExaTnPmpsVisitor.cpp:1060-1072), at most min(2 chiL chiR, 2 kap) of them, the
+// reference's new Kraus extent (:985-996).
+constexpr double KRAUS_EPS = 1e-9;
+
+int apply_kraus(hipmps_state_s* s, int q, int n_ops, const cd* K) {
+  hipmps_engine E = s->e;
+  const hipStream_t st = E->stream;
+  RC_TRY(state_flush(s));  // every gate before the channel, pending 1-qubit gates on q included
+  const int cl = s->chi[q], cr = s->chi[q + 1], kap = s->kap[q];
+  const int rows = kap * n_ops;
+  if (rows > 4096) return fail(HIPMPS_E_UNSUPPORTED, "Kraus leg above 4096 rows");
+  const int r_pad = (rows + JP - 1) / JP * JP;
+  const size_t c = (size_t)2 * cl * cr;
+  size_t off = 0;
+  const size_t off_x = off;
+  off = align_up(off + (size_t)r_pad * c * 16);
+  const size_t off_v = off;
+  off = align_up(off + (size_t)r_pad * (8 * 5 + 4));
+  const size_t off_k = off;
+  off = align_up(off + (size_t)n_ops * 4 * 16);
+  RC_TRY(E->ws.ensure(off));
+  char* ws = (char*)E->ws.p;
+  double2* X = (double2*)(ws + off_x);
+  RC_TRY(E->small.ensure(align_up(3 * sizeof(int)) + WS_N * sizeof(double)));
+  int* d_counts = (int*)E->small.p;  // rotated, visited
+  int* d_K = d_counts + 2;
+  double* d_w = (double*)((char*)E->small.p + align_up(3 * sizeof(int)));
+  const size_t desc_bytes = align_up(sizeof(SplitTask)) + align_up(sizeof(JacTask));
+  RC_TRY(E->desc.ensure(desc_bytes));
+  const size_t kbytes = (size_t)n_ops * 4 * sizeof(double2);
+  RC_TRY(E->hdesc.ensure(desc_bytes + kbytes + align_up(3 * sizeof(int)) + WS_N * sizeof(double)));
+  char* hd = (char*)E->hdesc.p;
+  SplitTask* hs = (SplitTask*)hd;
+  JacTask* hj = (JacTask*)(hd + align_up(sizeof(SplitTask)));
+  double2* hk = (double2*)(hd + desc_bytes);
+  int* hcount = (int*)(hd + desc_bytes + kbytes);
+  double* hw = (double*)(hd + desc_bytes + kbytes + align_up(3 * sizeof(int)));
+  for (int t = 0; t < 4 * n_ops; ++t) hk[t] = make_double2(K[t].real(), K[t].imag());
+  memset(hs, 0, sizeof(SplitTask));
+  char* v = ws + off_v;
+  hs->Y = X;
+  hs->r = rows;
+  hs->c = (int)c;
+  hs->ldy = (int)c;
+  hs->m = rows;
+  hs->n = (int)c;
+  hs->rownorm = (double*)v;
+  hs->sig = (double*)(v + 8 * r_pad);
+  hs->scale_y = (double*)(v + 16 * r_pad);
+  hs->scale_w = (double*)(v + 24 * r_pad);
+  hs->inv_sig = (double*)(v + 32 * r_pad);
+  hs->perm = (int*)(v + 40 * r_pad);
+  hs->K = d_K;
+  hs->wstats = d_w;
+  hs->cutoff = KRAUS_EPS;
+  hs->max_bond = (int)std::min<size_t>(c, (size_t)2 * kap);
+  hs->pn_l2 = s->t.partial_norm == HIPMPS_PN_L2;
+  hs->site_fp32 = s->fp32;
+  hs->keep_rule = 1;
+  memset(hj, 0, sizeof(JacTask));
+  hj->Y = X;
+  hj->ldy = (int)c;
+  hj->r = rows;
+  hj->r_pad = r_pad;
+  hj->c = (int)c;
+  hj->c_tot = (int)c;
+  hj->nb = r_pad / JB;
+  hj->pair_begin = 0;
+  hj->counter = d_counts;
+  hj->visited = d_counts + 1;
+  hj->tol = s->fp32 ? E->tol_mixed : E->tol;
+  char* dd = (char*)E->desc.p;
+  const SplitTask* dsp = (const SplitTask*)dd;
+  const JacTask* djt = (const JacTask*)(dd + align_up(sizeof(SplitTask)));
+  HIP_TRY(hipMemcpyAsync(dd, hd, desc_bytes, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(ws + off_k, hk, kbytes, hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_kraus_expand(s->sites[q].p, s->fp32, cl, kap, cr, (const double2*)(ws + off_k), n_ops, X, r_pad, st));
+  bool converged = false;
+  for (int sweep = 0; sweep < E->max_sweeps && !converged; ++sweep) {
+    HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * sizeof(int), st));
+    for (int round = 0; round < hj->nb - 1; ++round)
+      HIP_TRY(launch_jacobi_round(djt, 1, hj->nb / 2, round, E->inner_sweeps, st));
+    HIP_TRY(hipMemcpyAsync(hcount, d_counts, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    converged = hcount[0] == 0;
+  }
+  if (!converged) s->status = HIPMPS_E_CONVERGENCE;
+  HIP_TRY(launch_rownorm(dsp, 1, rows, st));
+  HIP_TRY(launch_select(dsp, 1, st));
+  HIP_TRY(hipMemcpyAsync(hcount + 2, d_K, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hw, d_w, WS_N * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const int Kn = hcount[2];
+  if (Kn < 1 || Kn > rows) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
+  if (hw[5] != 0.0 || !std::isfinite(hw[0] + hw[1])) s->status = HIPMPS_E_NUMERIC;
+  RC_TRY(ensure_site(s, q, (size_t)cl * 2 * Kn * cr, false, nullptr));
+  HIP_TRY(launch_kraus_gather(X, (int)c, (const int*)(v + 40 * r_pad), d_K, Kn, s->sites[q].p, s->fp32, cl, cr, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  s->kap[q] = Kn;
+  s->kraus_ops++;
   return 0;
 }
 
@@ -827,6 +977,7 @@ struct EnvJob {
   double2* Eout;
   int z;              // 1: apply diag(d0, d1) on the physical index of this site
   double d0 = 1.0, d1 = -1.0;
+  int kap = 1;        // Kraus leg of the site, traced with the physical index
 };
 
 // left step: E'(cr x cr) = A^H D (E A), D = diag(+-1) on the physical index when z
@@ -841,10 +992,11 @@ int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
     auto& j = jobs[q];
     GemmTask& a = h[q];
     memset(&a, 0, sizeof(a));
+    const int d = 2 * j.kap;  // physical index and Kraus leg, (p, k) with k fastest
     a.A = j.Ein; a.lda = j.cl;
-    a.B = j.A; a.ldb = 2 * j.cr;
-    a.C = j.X; a.ldc = 2 * j.cr;
-    a.M = j.cl; a.N = 2 * j.cr; a.K = j.cl;
+    a.B = j.A; a.ldb = d * j.cr;
+    a.C = j.X; a.ldc = d * j.cr;
+    a.M = j.cl; a.N = d * j.cr; a.K = j.cl;
     a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
     a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
     GemmTask& b = h[N + q];
@@ -852,8 +1004,8 @@ int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
     b.A = j.A; b.lda = j.cr;
     b.B = j.X; b.ldb = j.cr;
     b.C = j.Eout; b.ldc = j.cr;
-    b.M = j.cr; b.N = j.cr; b.K = 2 * j.cl;
-    b.sign_div = j.z ? 1 : 0;
+    b.M = j.cr; b.N = j.cr; b.K = d * j.cl;
+    b.sign_div = j.z ? j.kap : 0;
     b.kd0 = j.d0;
     b.kd1 = j.d1;
     b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
@@ -879,19 +1031,20 @@ int env_right(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
     auto& j = jobs[q];
     GemmTask& a = h[q];
     memset(&a, 0, sizeof(a));
-    a.A = j.A; a.lda = j.cr;                 // (2cl) x cr
+    const int d = 2 * j.kap;
+    a.A = j.A; a.lda = j.cr;                 // (d cl) x cr
     a.B = j.Ein; a.ldb = j.cr;               // R^T
     a.C = j.X; a.ldc = j.cr;
-    a.M = 2 * j.cl; a.N = j.cr; a.K = j.cr;
+    a.M = d * j.cl; a.N = j.cr; a.K = j.cr;
     a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
     a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
     GemmTask& b = h[N + q];
     memset(&b, 0, sizeof(b));
-    b.A = j.A; b.lda = 2 * j.cr;             // conj, cl x 2cr
-    b.B = j.X; b.ldb = 2 * j.cr;             // X viewed cl x 2cr, transposed
+    b.A = j.A; b.lda = d * j.cr;             // conj, cl x (d cr)
+    b.B = j.X; b.ldb = d * j.cr;             // X viewed cl x (d cr), transposed
     b.C = j.Eout; b.ldc = j.cl;
-    b.M = j.cl; b.N = j.cl; b.K = 2 * j.cr;
-    b.sign_div = j.z ? j.cr : 0;
+    b.M = j.cl; b.N = j.cl; b.K = d * j.cr;
+    b.sign_div = j.z ? j.kap * j.cr : 0;
     b.kd0 = j.d0;
     b.kd1 = j.d1;
     b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
@@ -927,7 +1080,7 @@ EnvLayout env_layout(const hipmps_state_s* s, size_t extra_mats, size_t extra_el
     off = align_up(off + (size_t)s->chi[k] * s->chi[k] * 16);
   }
   L.offX = off;
-  off = align_up(off + align_up((size_t)cmax * 2 * cmax * 16) * std::max<size_t>(1, extra_mats));
+  off = align_up(off + align_up((size_t)cmax * 2 * max_kap(s) * cmax * 16) * std::max<size_t>(1, extra_mats));
   L.offT = off;
   off = align_up(off + std::max<size_t>(extra_elems, 1) * 16);
   L.total = off;
@@ -943,6 +1096,7 @@ int compute_left_envs_diag(hipmps_state_s* s, const EnvLayout& L, const std::vec
     std::vector<EnvJob> jobs(1);
     jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offL[k]),
                (double2*)(base + L.offX), (double2*)(base + L.offL[k + 1]), 0};
+    jobs[0].kap = s->kap[k];
     if (!diag.empty() && !(diag[2 * k] == 1.0 && diag[2 * k + 1] == 1.0)) {
       jobs[0].z = 1;
       jobs[0].d0 = diag[2 * k];
@@ -971,6 +1125,7 @@ int compute_right_envs(hipmps_state_s* s, const EnvLayout& L) {
     std::vector<EnvJob> jobs(1);
     jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offR[k + 1]),
                (double2*)(base + L.offX), (double2*)(base + L.offR[k]), 0};
+    jobs[0].kap = s->kap[k];
     RC_TRY(env_right(E, jobs, s->fp32));
   }
   return 0;
@@ -1028,10 +1183,12 @@ int measure_z_all(hipmps_state_s* s, double* out) {
   RC_TRY(compute_right_envs(s, L));
   // M_i = env update of L_i with Z at i (batched); X scratch per job
   std::vector<EnvJob> jobs;
-  size_t xstride = align_up((size_t)cmax * 2 * cmax * 16);
-  for (int i = 0; i < n; ++i)
+  size_t xstride = align_up((size_t)cmax * 2 * max_kap(s) * cmax * 16);
+  for (int i = 0; i < n; ++i) {
     jobs.push_back({s->sites[i].p, s->chi[i], s->chi[i + 1], (const double2*)(base + L.offL[i]),
                     (double2*)(base + L.offX + i * xstride), (double2*)(base + L.offT + offM[i]), 1});
+    jobs.back().kap = s->kap[i];
+  }
   RC_TRY(env_left(E, jobs, s->fp32));
   std::vector<const double2*> A, B;
   std::vector<int> dims;
@@ -1053,7 +1210,7 @@ int measure_zz(hipmps_state_s* s, const int* pairs, int k, double* out) {
   int cmax = 1;
   for (int b = 0; b <= n; ++b) cmax = std::max(cmax, s->chi[b]);
   size_t mat = align_up((size_t)cmax * cmax * 16);
-  size_t xmat = align_up((size_t)cmax * 2 * cmax * 16);
+  size_t xmat = align_up((size_t)cmax * 2 * max_kap(s) * cmax * 16);
   EnvLayout L = env_layout(s, 1, 0);
   size_t per_pair = 2 * mat + xmat;
   size_t total = L.total + per_pair * (size_t)k + 4096 + (size_t)k * 64;
@@ -1088,6 +1245,7 @@ int measure_zz(hipmps_state_s* s, const int* pairs, int k, double* out) {
       double2* eout = (double2*)(buf + mat * (1 - p.pp));
       int z = (p.cur == p.i || p.cur == p.j) ? 1 : 0;
       jobs.push_back({s->sites[p.cur].p, s->chi[p.cur], s->chi[p.cur + 1], ein, (double2*)(buf + 2 * mat), eout, z});
+      jobs.back().kap = s->kap[p.cur];
       who.push_back(t);
     }
     if (!any) break;
@@ -1140,6 +1298,8 @@ int measure_diag(hipmps_state_s* s, const std::vector<double>& diag, double* out
 
 // amplitude / slice chain
 int chain_contract(hipmps_state_s* s, int rows0, const int8_t* d_bits, const int* slice_bits, std::vector<double2>& out) {
+  if (is_purified(s))
+    return fail(HIPMPS_E_UNSUPPORTED, "amplitudes are not defined for a purified (mixed) state");
   hipmps_engine E = s->e;
   const int n = s->n;
   // rows after each site
@@ -1350,6 +1510,23 @@ int hipmps_flush(hipmps_state s) {
   return s->status;
 }
 
+int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats) {
+  if (!s || !mats || q < 0 || q >= s->n || n_ops < 1 || n_ops > HIPMPS_MAX_KRAUS_OPS)
+    return fail(HIPMPS_E_ARG, "bad arguments");
+  if (!all_finite(mats, (size_t)8 * n_ops)) return fail(HIPMPS_E_NUMERIC, "Kraus operator holds NaN or Inf");
+  std::vector<cd> K(4 * (size_t)n_ops);
+  for (size_t t = 0; t < K.size(); ++t) K[t] = cd(mats[2 * t], mats[2 * t + 1]);
+  RC_TRY(set_device(s->e));
+  return apply_kraus(s, q, n_ops, K.data());
+}
+
+int hipmps_kraus_dims(hipmps_state s, int* out) {
+  if (!s || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(state_flush(s));
+  for (int k = 0; k < s->n; ++k) out[k] = s->kap[k];
+  return 0;
+}
+
 int hipmps_bond_dims(hipmps_state s, int* out) {
   if (!s || !out) return fail(HIPMPS_E_ARG, "null argument");
   RC_TRY(state_flush(s));
@@ -1418,7 +1595,7 @@ int hipmps_state_clone(hipmps_state src, hipmps_state* out) {
   hipmps_state dst = nullptr;
   RC_TRY(hipmps_state_create(src->e, src->n, &src->t, &dst));
   for (int k = 0; k < src->n; ++k) {
-    size_t n = (size_t)src->chi[k] * 2 * src->chi[k + 1];
+    size_t n = site_elems(src, k);
     int rc = ensure_site(dst, k, n, false, nullptr);
     if (rc) {
       hipmps_state_destroy(dst);
@@ -1432,6 +1609,7 @@ int hipmps_state_clone(hipmps_state src, hipmps_state* out) {
   }
   HIP_TRY(hipStreamSynchronize(src->e->stream));
   dst->chi = src->chi;
+  dst->kap = src->kap;
   dst->discarded = src->discarded;
   dst->fid = src->fid;
   dst->maxb = src->maxb;
@@ -1496,7 +1674,7 @@ int hipmps_state_vector(hipmps_state s, double* out, size_t cap) {
 int hipmps_get_site(hipmps_state s, int k, double* out, size_t cap) {
   if (!s || !out || k < 0 || k >= s->n) return fail(HIPMPS_E_ARG, "bad arguments");
   RC_TRY(state_flush(s));
-  size_t n = (size_t)s->chi[k] * 2 * s->chi[k + 1];
+  size_t n = site_elems(s, k);
   if (cap < n) return fail(HIPMPS_E_ARG, "output buffer too small");
   hipmps_engine E = s->e;
   if (s->fp32) {
@@ -1519,6 +1697,7 @@ int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* i
   RC_TRY(ensure_site(s, k, n, false, nullptr));
   s->chi[k] = chi_l;
   s->chi[k + 1] = chi_r;
+  s->kap[k] = 1;
   s->maxb = std::max(s->maxb, std::max(chi_l, chi_r));
   return upload_site_host(s, k, reinterpret_cast<const cd*>(in), n);
 }

@@ -70,7 +70,8 @@ struct SplitTask {
   int absorb;            // 0 sqrt split, 1 all to the right factor
   int renorm;
   int site_fp32;         // site storage is complex64
-  int pad;
+  int keep_rule;         // 0: keep i + 1 for the first sub-threshold index i (bond);
+                         // 1: keep i (Kraus leg, ExaTnPmpsVisitor.cpp:1060-1072)
 };
 
 // Generic batched complex GEMM: C (M x N) = rowscale . op(A) op(B) . colscale
@@ -163,5 +164,18 @@ hipError_t launch_trace_prod(const double2* const* d_a, const double2* const* d_
 hipError_t launch_check_finite(const void* p, size_t n_complex, int fp32, int* d_flag,
                                hipStream_t s);
 hipError_t launch_fill_identity1(double2* p, hipStream_t s);
+
+// ---- purified sites (Kraus leg), kraus.hip ------------------------------------------------
+// dst[l][b][a][m] = src[l][a][b][m] for m < M (M = *m_dyn when m_dyn is not null, at most
+// m_max); element type complex128 (fp32 = 0) or complex64.
+hipError_t launch_swap_mid(const void* src, void* dst, int L, int A, int B, int m_max, const int* m_dyn,
+                           int fp32, hipStream_t s);
+// X[k * M + o][(l * 2 + p) * cr + r] = sum_p' K_o[p][p'] A[l][p'][k][r] for the site A
+// (cl x 2 x kap x cr), rows kap * M .. rows_pad - 1 zero; K: M row-major 2 x 2 matrices.
+hipError_t launch_kraus_expand(const void* site, int fp32, int cl, int kap, int cr, const double2* d_kraus,
+                               int n_ops, double2* X, int rows_pad, hipStream_t s);
+// site[l][p][j][r] = Y[perm[j]][(l * 2 + p) * cr + r] for j < *K (at most k_max).
+hipError_t launch_kraus_gather(const double2* Y, int ldy, const int* perm, const int* K, int k_max, void* site,
+                               int fp32, int cl, int cr, hipStream_t s);
 
 }  // namespace hipmps

@@ -614,12 +614,16 @@ __global__ __launch_bounds__(256) void k_select(const SplitTask* sp) {
     }
   }
   // first sub-threshold index
+  // partial norm of index i: across a sqrt(S) split each factor carries sqrt(sigma_i), so
+  // sigma_i (squared norm) or sqrt(sigma_i) (2-norm); a Kraus vector sigma_i v_i carries
+  // sigma_i^2 or sigma_i
   for (int i = tid; i < r; i += 256) {
-    double pn = S.pn_l2 ? sqrt(key[i]) : key[i];
+    const double sg = key[i];
+    double pn = S.keep_rule ? (S.pn_l2 ? sg : sg * sg) : (S.pn_l2 ? sqrt(sg) : sg);
     if (pn < S.cutoff) atomicMin(&s_first, i);
   }
   __syncthreads();
-  int K = s_first < r ? s_first + 1 : r;
+  int K = s_first < r ? (S.keep_rule ? s_first : s_first + 1) : r;
   if (K > S.max_bond) K = S.max_bond;
   if (K < 1) K = 1;
   // kept / discarded weights (FP64)

@@ -285,11 +285,99 @@ class MpsState:
         return out
 
     def get_site(self, k: int) -> np.ndarray:
+        """Site k as (chi_l, 2, chi_r), or (chi_l, 2, kap, chi_r) once it has a Kraus leg."""
         b = self.bonds()
-        out = np.zeros(b[k] * 2 * b[k + 1], dtype=np.complex128)
+        kap = int(self.kraus_dims()[k])
+        out = np.zeros(b[k] * 2 * kap * b[k + 1], dtype=np.complex128)
         check(lib().hipmps_get_site(self.h, k, out.view(np.float64).ctypes.data_as(_dp), out.size))
-        return out.reshape(b[k], 2, b[k + 1])
+        return out.reshape(b[k], 2, b[k + 1]) if kap == 1 else out.reshape(b[k], 2, kap, b[k + 1])
+
+    def apply_kraus(self, q: int, ops):
+        """Noise channel rho -> sum_o K_o rho K_o^H on qubit q (2 x 2 Kraus operators)."""
+        k = np.ascontiguousarray(np.asarray(ops, dtype=np.complex128).reshape(-1, 2, 2))
+        check(lib().hipmps_apply_kraus(self.h, int(q), len(k), k.view(np.float64).ctypes.data_as(_dp)))
+        return self
+
+    def clone(self) -> "MpsState":
+        """Deep copy on the same engine (hipmps_state_clone)."""
+        h = ctypes.c_void_p()
+        check(lib().hipmps_state_clone(self.h, ctypes.byref(h)))
+        c = MpsState.__new__(MpsState)
+        c.e, c.n, c.trunc, c.h = self.e, self.n, self.trunc, h
+        self.e._states.add(c)
+        return c
+
+    def expval_diag(self, diag) -> float:
+        """<(x)_k diag(d[2k], d[2k+1])>, a (2n,) array of per-site diagonals."""
+        d = np.ascontiguousarray(np.asarray(diag, dtype=np.float64).reshape(-1))
+        v = ctypes.c_double()
+        check(lib().hipmps_expval_diag(self.h, d.ctypes.data_as(_dp), ctypes.byref(v)))
+        return v.value
+
+    def outcome_probabilities(self, qubits) -> np.ndarray:
+        """P(outcome) over `qubits` (index sum_i b_i 2^i, b_i the bit of qubits[i]): one
+        projector string per outcome through the environment contraction, for pure and
+        purified states alike."""
+        k = len(qubits)
+        if k > 16:
+            raise ValueError("at most 16 measured qubits")
+        out = np.zeros(1 << k)
+        for x in range(1 << k):
+            d = np.ones(2 * self.n)
+            for i, q in enumerate(qubits):
+                b = (x >> i) & 1
+                d[2 * q + (1 - b)] = 0.0
+            out[x] = self.expval_diag(d)
+        return out
+
+    def kraus_dims(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.int32)
+        check(lib().hipmps_kraus_dims(self.h, out.ctypes.data_as(_ip)))
+        return out
 
     def set_site(self, k: int, a: np.ndarray):
         a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
         check(lib().hipmps_set_site(self.h, k, a.shape[0], a.shape[2], a.view(np.float64).ctypes.data_as(_dp)))
+
+
+def run_noisy(engine: Engine, n: int, gates, noise=None, measured=(), shots: int = 1024, seed=None,
+              precision: str = "fp64"):
+    """A noisy circuit the way the reference's purified-MPS visitor runs one
+    (ExaTnPmpsVisitor.cpp): each gate followed by the single-qubit channels the noise model
+    gives for it (:775-829), bonds cut at 1e-9 with no cap (truncateSvdTensors' default,
+    :1255), then the outcome distribution of the measured qubits and `shots` strings sampled
+    from it with the model's readout errors (finalize, :657-782; the default of 1024 shots
+    for a noisy run, :490).  String character i is the outcome of measured[i].
+
+    noise: None or an object with channels(gate) -> [(qubit, Kraus ops)] and
+    readout_error(qubit) -> (P(0 | 1), P(1 | 0)) (tnqvm_amd.noise.NoiseModel)."""
+    s = MpsState(engine, n, cutoff=1e-9, precision=precision)
+    try:
+        for g in gates:
+            if g["name"] in ("Measure", "I"):
+                continue
+            s.run([g])
+            if noise is not None:
+                for q, ops in noise.channels(g):
+                    s.apply_kraus(q, ops)
+        measured = list(measured)
+        probs = s.outcome_probabilities(measured) if measured else np.ones(1)
+        rng = np.random.default_rng(seed)
+        p = np.clip(probs, 0.0, None)
+        picks = rng.choice(len(p), size=shots, p=p / p.sum()) if measured else np.zeros(0, dtype=int)
+        counts = {}
+        for x in picks:
+            bits = []
+            for i, q in enumerate(measured):
+                b = (int(x) >> i) & 1
+                if noise is not None:
+                    p01, p10 = noise.readout_error(q)  # P(read 0 | 1), P(read 1 | 0)
+                    if rng.random() < (p01 if b else p10):
+                        b ^= 1
+                bits.append("1" if b else "0")
+            key = "".join(bits)
+            counts[key] = counts.get(key, 0) + 1
+        return {"probabilities": probs, "counts": counts, "expval_z": s.expval_z_all(),
+                "bonds": s.bonds(), "kraus_dims": s.kraus_dims(), "trace": s.norm()}
+    finally:
+        s.close()
