@@ -405,6 +405,113 @@ CASE(accelerator_state, true) {
   EXPECT_NEAR(std::abs(sv[3]), 1.0 / std::sqrt(2.0), 1e-14);
 }
 
+// ------------------------------------------------------------------ hip-pmps (noisy)
+// ExaTnPmpsTester.cpp cases.  The reference builds its model from the ibmqx2 backend JSON
+// through XACC; here a model with the same structure (amplitude damping and dephasing after
+// every non-virtual gate, readout errors) is given directly as the noise-model option.
+class DampingModel : public xacc::NoiseModel {
+ public:
+  DampingModel(double gamma, double lam, double ro01, double ro10) : g_(gamma), l_(lam), ro_{ro01, ro10} {}
+  std::vector<xacc::NoiseChannelKraus> getNoiseChannels(xacc::quantum::Gate& gate) override {
+    std::vector<xacc::NoiseChannelKraus> out;
+    if (gate.name() == "Rz" || gate.name() == "Measure") return out;
+    using C = std::complex<double>;
+    for (size_t q : gate.bits()) {
+      xacc::NoiseChannelKraus ad, pd;
+      ad.noise_qubits = {q};
+      ad.mats = {{{C(1), C(0)}, {C(0), C(std::sqrt(1 - g_))}}, {{C(0), C(std::sqrt(g_))}, {C(0), C(0)}}};
+      pd.noise_qubits = {q};
+      pd.mats = {{{C(1), C(0)}, {C(0), C(std::sqrt(1 - l_))}}, {{C(0), C(0)}, {C(0), C(std::sqrt(l_))}}};
+      out.push_back(ad);
+      out.push_back(pd);
+    }
+    return out;
+  }
+  std::pair<double, double> readoutError(size_t) override { return ro_; }
+
+ private:
+  double g_, l_;
+  std::pair<double, double> ro_;
+};
+
+std::shared_ptr<Accelerator> pmps(xacc::NoiseModel* model, int shots = 1024) {
+  HeterogeneousMap m{{"tnqvm-visitor", std::string("hip-pmps")}};
+  if (model) m.insert("noise-model", model);
+  m.insert("shots", shots);
+  m.insert("seed", 7);
+  return getAccelerator("tnqvm", m);
+}
+
+// ExaTnPmpsTester.cpp:20-42: 100 X gates with damping leave a visible "1" population
+CASE(pmps_check_simple, true) {
+  DampingModel model(0.002, 0.001, 0.05, 0.02);
+  auto f = compile1(R"(__qpu__ void test1(qbit q) {
+    for (int i = 0; i < 100; i++) { X(q[0]); }
+    Measure(q[0]);
+  })");
+  auto buffer = std::make_shared<AcceleratorBuffer>(1);
+  pmps(&model)->execute(buffer, f);
+  EXPECT_TRUE(buffer->computeMeasurementProbability("1") > 0.01);
+  // X^100 = I; damping after each X: P(1) = sum over the alternation, exactly from rho
+  auto diag = (*buffer)["density-matrix-diagonal"].as<std::vector<double>>();
+  EXPECT_TRUE(diag[1] > 0.01 && diag[1] < 0.5);
+  EXPECT_NEAR(diag[0] + diag[1], 1.0, 1e-12);
+}
+
+// :44-67 and :69-89: |111> and a Bell pair, most likely but not certain under noise
+CASE(pmps_check_multiple_qubits_and_bell, true) {
+  DampingModel model(0.01, 0.005, 0.03, 0.01);
+  auto f = compile1(R"(__qpu__ void test2(qbit q) {
+    X(q[0]); X(q[1]); X(q[2]);
+    Measure(q[0]); Measure(q[1]); Measure(q[2]);
+  })");
+  auto b3 = std::make_shared<AcceleratorBuffer>(3);
+  pmps(&model)->execute(b3, f);
+  const double p111 = b3->computeMeasurementProbability("111");
+  EXPECT_TRUE(p111 > 0.8 && p111 < 0.99);
+  auto g = compile1(R"(__qpu__ void testBell(qbit q) {
+    X(q[0]); CX(q[0], q[1]);
+    Measure(q[0]); Measure(q[1]);
+  })");
+  auto b2 = std::make_shared<AcceleratorBuffer>(2);
+  pmps(&model)->execute(b2, g);
+  const double p11 = b2->computeMeasurementProbability("11");
+  EXPECT_TRUE(p11 > 0.8 && p11 < 0.99);
+  // without readout errors and noise, the distribution is exact
+  auto b2c = std::make_shared<AcceleratorBuffer>(2);
+  pmps(nullptr)->execute(b2c, g);
+  auto d = (*b2c)["density-matrix-diagonal"].as<std::vector<double>>();
+  EXPECT_NEAR(d[3], 1.0, 1e-12);
+}
+
+// :91-115: a 100-qubit GHZ chain, 6 of its qubits measured
+CASE(pmps_check_many_qubits, true) {
+  auto f = compile1(R"(__qpu__ void testGHz(qbit q) {
+    H(q[0]);
+    for (int i = 0; i < 99; i++) { CX(q[i], q[i + 1]); }
+    Measure(q[0]); Measure(q[6]); Measure(q[8]); Measure(q[36]); Measure(q[72]); Measure(q[98]);
+  })");
+  auto buffer = std::make_shared<AcceleratorBuffer>(100);
+  pmps(nullptr)->execute(buffer, f);
+  EXPECT_NEAR(buffer->computeMeasurementProbability("111111"), 0.5, 0.1);
+  EXPECT_NEAR(buffer->computeMeasurementProbability("000000"), 0.5, 0.1);
+  auto d = (*buffer)["density-matrix-diagonal"].as<std::vector<double>>();
+  EXPECT_NEAR(d[0], 0.5, 1e-10);
+  EXPECT_NEAR(d[63], 0.5, 1e-10);
+}
+
+// a routed gate: lnn-transform swaps are visited, and carry noise like any other gate
+CASE(pmps_routed_gate_noise, true) {
+  DampingModel model(0.05, 0.0, 0.0, 0.0);
+  auto f = compile1(R"(__qpu__ void r(qbit q) { X(q[0]); CX(q[0], q[3]); Measure(q[3]); })");
+  auto b = std::make_shared<AcceleratorBuffer>(4);
+  pmps(&model)->execute(b, f);
+  auto d = (*b)["density-matrix-diagonal"].as<std::vector<double>>();
+  EXPECT_TRUE(d[1] < 0.95 && d[1] > 0.5);
+  auto kd = (*b)["kraus-dims"].as<std::vector<int>>();
+  EXPECT_TRUE(kd[3] > 1);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {

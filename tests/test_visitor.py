@@ -2,7 +2,8 @@
 
 CPU cases: XASM front end, lnn-transform routing, option handling.
 GPU cases: the exatn-mps gtest cases (gates, routing, sampling, seed determinism, norm,
-closed forms, amplitudes / slices, VQE mode, accelerator state) against hip-mps.
+closed forms, amplitudes / slices, VQE mode, accelerator state) against hip-mps, and the
+exatn-pmps gtest cases (ExaTnPmpsTester.cpp) against hip-pmps.
 """
 import os
 import subprocess

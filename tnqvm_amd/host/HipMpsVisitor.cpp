@@ -45,14 +45,10 @@ void HipMpsVisitor::check(int rc, const char* what) const {
   if (rc != 0) xacc::error(std::string("hip-mps: ") + what + " failed (" + std::to_string(rc) + "): " + hipmps_last_error());
 }
 
-void HipMpsVisitor::initialize(std::shared_ptr<AcceleratorBuffer> in_buffer, int nbShots) {
-  buffer = std::move(in_buffer);
-  m_n = buffer->size();
-  m_shots = nbShots;
-  m_measured.clear();
+hipmps_trunc_opts HipMpsVisitor::trunc_options(double default_cutoff) const {
   hipmps_trunc_opts t{};
-  t.svd_cutoff = DBL_MIN;   // ExaTnMpsVisitor.cpp:257-263
-  t.max_bond_dim = 0;       // INT_MAX-1, :265-271
+  t.svd_cutoff = default_cutoff;
+  t.max_bond_dim = 0;  // INT_MAX-1, ExaTnMpsVisitor.cpp:265-271
   if (options.keyExists<double>("svd-cutoff")) t.svd_cutoff = options.get<double>("svd-cutoff");
   if (options.keyExists<int>("max-bond-dim")) t.max_bond_dim = options.get<int>("max-bond-dim");
   if (options.stringExists("precision")) {
@@ -64,6 +60,15 @@ void HipMpsVisitor::initialize(std::shared_ptr<AcceleratorBuffer> in_buffer, int
   if (options.keyExists<bool>("renormalize")) t.renormalize = options.get<bool>("renormalize") ? 1 : 0;
   if (options.stringExists("partial-norm")) t.partial_norm = options.getString("partial-norm") == "l2" ? HIPMPS_PN_L2 : HIPMPS_PN_SQUARED;
   if (options.stringExists("absorb")) t.absorb = options.getString("absorb") == "right" ? HIPMPS_ABSORB_RIGHT : HIPMPS_ABSORB_SQRT;
+  return t;
+}
+
+void HipMpsVisitor::initialize(std::shared_ptr<AcceleratorBuffer> in_buffer, int nbShots) {
+  buffer = std::move(in_buffer);
+  m_n = buffer->size();
+  m_shots = nbShots;
+  m_measured.clear();
+  hipmps_trunc_opts t = trunc_options(DBL_MIN);  // ExaTnMpsVisitor.cpp:257-263
   const int device = options.keyExists<int>("device") ? options.get<int>("device") : 0;
   if (m_state) {
     hipmps_state_destroy(m_state);
@@ -79,6 +84,7 @@ void HipMpsVisitor::apply(xacc::Instruction& g) {
   const int k = hipmps_gate_matrix(g.name().c_str(), p.empty() ? nullptr : p.data(), (int)p.size(), m);
   if (k < 0) xacc::error("hip-mps: no matrix for gate " + g.name());
   apply_matrix(g.bits(), m);
+  after_gate(g);
 }
 
 // Non-neighbouring 2-qubit gates are routed with the meet-in-the-middle swap scheme of

@@ -72,12 +72,17 @@ class HipMpsVisitor : public TNQVMVisitor {
   // the engine shared by all hip-mps visitors of this process (one per device)
   static hipmps_engine engine(int device);
 
- private:
+ protected:
   void apply(xacc::Instruction& g);
+  // called after every gate has been queued on the state (noise channels of the purified-MPS
+  // visitor); nothing here
+  virtual void after_gate(xacc::Instruction&) {}
   void apply_matrix(const std::vector<size_t>& bits, const double* m);
   void check(int rc, const char* what) const;
   std::vector<std::string> sample_state_vector(int shots);
   std::vector<std::string> sample_sequential(int shots);
+  // truncation options from the shared option keys, with the given default cutoff
+  hipmps_trunc_opts trunc_options(double default_cutoff) const;
 
   hipmps_state m_state = nullptr;
   int m_n = 0;

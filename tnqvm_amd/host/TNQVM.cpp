@@ -2,6 +2,7 @@
 #include "TNQVM.hpp"
 
 #include "HipMpsVisitor.hpp"
+#include "HipPmpsVisitor.hpp"
 #include "RandomEngine.hpp"
 
 namespace tnqvm {
@@ -12,8 +13,9 @@ namespace {
 int shotCountOption(const HeterogeneousMap& o) { return o.keyExists<int>("shots") ? o.get<int>("shots") : -1; }
 
 bool isMpsVisitor(const std::string& n) {
-  // visitors that need nearest-neighbour gates (TNQVM.cpp:119-124); hip-mps routes itself
-  return n == "itensor-mps" || n == "exatn-mps" || n == "exatn-pmps";
+  // visitors that need nearest-neighbour gates (TNQVM.cpp:119-124); hip-mps routes itself,
+  // hip-pmps takes the routed circuit like exatn-pmps, so that routing swaps carry noise
+  return n == "itensor-mps" || n == "exatn-mps" || n == "exatn-pmps" || n == "hip-pmps";
 }
 
 void walk(const std::shared_ptr<CompositeInstruction>& k, const std::shared_ptr<TNQVMVisitor>& v) {
@@ -185,6 +187,7 @@ struct Activator {
   Activator() {
     xacc::registerService<xacc::Accelerator>("tnqvm", std::make_shared<TNQVM>());
     xacc::registerService<TNQVMVisitor>("hip-mps", std::make_shared<HipMpsVisitor>());
+    xacc::registerService<TNQVMVisitor>("hip-pmps", std::make_shared<HipPmpsVisitor>());
     xacc::registerService<xacc::IRTransformation>("lnn-transform", std::make_shared<NearestNeighborTransform>());
   }
 };

@@ -61,6 +61,19 @@ class HeterogeneousMap {
     return std::any_cast<T>(it->second);
   }
   std::string getString(const std::string& key) const { return get<std::string>(key); }
+  // options that carry an object by pointer (XACC's pointerLikeExists / getPointerLike):
+  // a raw T* or a std::shared_ptr<T>
+  template <typename T>
+  bool pointerLikeExists(const std::string& key) const {
+    return keyExists<T*>(key) || keyExists<std::shared_ptr<T>>(key);
+  }
+  template <typename T>
+  T* getPointerLike(const std::string& key) const {
+    if (keyExists<T*>(key)) return get<T*>(key);
+    if (keyExists<std::shared_ptr<T>>(key)) return get<std::shared_ptr<T>>(key).get();
+    error("HeterogeneousMap: no pointer-like value for key " + key);
+    return nullptr;
+  }
   void merge(const HeterogeneousMap& o) {
     for (auto& kv : o.m_) m_[kv.first] = kv.second;
   }
@@ -360,6 +373,22 @@ inline std::shared_ptr<CompositeInstruction> CompositeInstruction::operator()(co
   }
   return out;
 }
+
+// ------------------------------------------------------------------ noise models
+// The interface the reference's purified-MPS visitor reads its noise from
+// (ExaTnPmpsVisitor.cpp:470-487, :775-829): per gate, Kraus channels on some qubits; per
+// qubit, the readout errors (P(measure 0 | prepared 1), P(measure 1 | prepared 0)).
+struct NoiseChannelKraus {
+  std::vector<size_t> noise_qubits;
+  std::vector<std::vector<std::vector<std::complex<double>>>> mats;  // Kraus operators, row-major
+};
+
+class NoiseModel {
+ public:
+  virtual ~NoiseModel() = default;
+  virtual std::vector<NoiseChannelKraus> getNoiseChannels(quantum::Gate& gate) = 0;
+  virtual std::pair<double, double> readoutError(size_t qubit) = 0;
+};
 
 // ------------------------------------------------------------------ services
 class Accelerator {
