@@ -288,7 +288,9 @@ def main():
     log(f"  {applies} applies in {total:.3f} s -> {value:.1f} applies/s; jacobi {prof['ms_jacobi']:.0f} ms, "
         f"rounds {prof['ms_jacobi_rounds']:.0f} ms, launches {prof['jacobi_launches']}")
 
-    # roofline of the dominant kernel (k_jacobi_round): executed MFMA flops / event-timed launch spans
+    # roofline of the dominant kernel (k_jacobi_round): executed MFMA flops / the HIP-event span of
+    # its rounds (the SVDs of a wave are dealt to two streams whose launches overlap, so the
+    # span is the union of the launch intervals, and span / launches is the time per launch)
     jac_tflops = prof["jacobi_flops"] / max(prof["ms_jacobi_rounds"], 1e-9) / 1e9
     avg_launch_us = 1e3 * prof["ms_jacobi_rounds"] / max(prof["jacobi_launches"], 1)
     # HBM bytes per launch: per-block-pair FETCH/WRITE from the committed rocprofv3 counter runs
@@ -302,6 +304,7 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(jac_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(jac_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_jacobi_round", "avg_launch_us": round(avg_launch_us, 2),
+                "timing": "union span of the rounds on two streams / launches",
                 "launches": prof["jacobi_launches"],
                 "flops_per_launch": prof["jacobi_flops"] / max(prof["jacobi_launches"], 1)}
     nominal_flops = 736.0 * CHI ** 3  # per apply at chi=256 (theta + gate + nominal SVD)

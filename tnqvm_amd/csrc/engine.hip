@@ -152,6 +152,13 @@ struct PendingCircuit;
 struct hipmps_engine_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // streams for the Jacobi rounds: the wave's SVDs are dealt to `jacobi_lanes` lists whose
+  // rounds run on their own streams (lane 0 on `stream`), so that the launch tail of one list
+  // overlaps the work of the others
+  static constexpr int MAX_LANES = 4;
+  hipStream_t lane_stream[MAX_LANES] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t lane_ev[MAX_LANES + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int jacobi_lanes = 2;
   size_t budget = 0;
   size_t used = 0;  // bytes of site storage held by states
   int max_sweeps = 40;
@@ -612,24 +619,61 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   int* hcounts = (int*)((char*)E->hdesc.p + desc_bytes);
   bool failed_conv = false;
   for (int sweep = 0; sweep < E->max_sweeps && !active.empty(); ++sweep) {
-    // active task list with pair prefix
+    // active task list with pair prefix; with several lanes, the items are dealt to lists
+    // of about equal work (largest first), each a prefix-indexed launch on its own stream
+    constexpr int ML = hipmps_engine_s::MAX_LANES;
     int nact = (int)active.size();
-    int pairs = 0, max_rounds = 0;
     JacTask* hja = (JacTask*)((char*)E->hdesc.p + desc_bytes + align_up(2 * N * sizeof(int)));
-    for (int a = 0; a < nact; ++a) {
-      hja[a] = hjt[active[a]];
-      hja[a].pair_begin = pairs;
-      pairs += hja[a].nb / 2;
-      max_rounds = std::max(max_rounds, hja[a].nb - 1);
+    const int lanes = std::max(1, std::min(E->jacobi_lanes, nact));
+    std::vector<int> order(active);
+    if (lanes > 1)
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hjt[a].nb > hjt[b].nb; });
+    std::vector<int> lane_items[ML];
+    long long lane_load[ML] = {0, 0, 0, 0};
+    for (int q : order) {
+      int l = 0;
+      for (int k = 1; k < lanes; ++k)
+        if (lane_load[k] < lane_load[l]) l = k;
+      lane_items[l].push_back(q);
+      lane_load[l] += (long long)(hjt[q].nb / 2) * hjt[q].c * (hjt[q].nb - 1);
+    }
+    int lane_begin[ML] = {0, 0, 0, 0}, lane_n[ML] = {0, 0, 0, 0}, lane_pairs[ML] = {0, 0, 0, 0},
+        lane_rounds[ML] = {0, 0, 0, 0};
+    {
+      int a = 0;
+      for (int l = 0; l < lanes; ++l) {
+        lane_begin[l] = a;
+        for (int q : lane_items[l]) {
+          hja[a] = hjt[q];
+          hja[a].pair_begin = lane_pairs[l];
+          lane_pairs[l] += hja[a].nb / 2;
+          lane_rounds[l] = std::max(lane_rounds[l], hja[a].nb - 1);
+          ++a;
+        }
+        lane_n[l] = a - lane_begin[l];
+      }
     }
     // reuse the JacTask region of the device descriptor block for the active list
     HIP_TRY(hipMemcpyAsync(djt, hja, nact * sizeof(JacTask), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * N * sizeof(int), st));
     if (E->timing) HIP_TRY(hipEventRecord(E->ev[4], st));
+    int max_rounds = 0;
+    for (int l = 0; l < lanes; ++l) max_rounds = std::max(max_rounds, lane_rounds[l]);
+    if (lanes > 1) {
+      HIP_TRY(hipEventRecord(E->lane_ev[0], st));
+      for (int l = 1; l < lanes; ++l) HIP_TRY(hipStreamWaitEvent(E->lane_stream[l], E->lane_ev[0], 0));
+    }
     for (int round = 0; round < max_rounds; ++round)
-      HIP_TRY(launch_jacobi_round(djt, nact, pairs, round, E->inner_sweeps, st));
+      for (int l = 0; l < lanes; ++l)
+        if (lane_n[l] > 0 && round < lane_rounds[l])
+          HIP_TRY(launch_jacobi_round(djt + lane_begin[l], lane_n[l], lane_pairs[l], round, E->inner_sweeps,
+                                      E->lane_stream[l]));
+    for (int l = 1; l < lanes; ++l) {
+      HIP_TRY(hipEventRecord(E->lane_ev[l], E->lane_stream[l]));
+      HIP_TRY(hipStreamWaitEvent(st, E->lane_ev[l], 0));
+    }
     if (E->timing) HIP_TRY(hipEventRecord(E->ev[5], st));
-    E->prof.jacobi_launches += max_rounds;
+    for (int l = 0; l < lanes; ++l) E->prof.jacobi_launches += lane_rounds[l];
     HIP_TRY(hipMemcpyAsync(hcounts, d_counts, 2 * N * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (E->timing) {
@@ -1385,6 +1429,17 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
   if (opts && opts->jacobi_tol > 0) e->tol = opts->jacobi_tol;
   if (opts && opts->jacobi_tol_mixed > 0) e->tol_mixed = opts->jacobi_tol_mixed;
   for (auto& ev : e->ev) (void)hipEventCreate(&ev);
+  if (const char* v = getenv("HIPMPS_JACOBI_LANES"))
+    e->jacobi_lanes = std::max(1, std::min(hipmps_engine_s::MAX_LANES, atoi(v)));
+  e->lane_stream[0] = e->stream;
+  bool ok = true;
+  for (int l = 1; l < hipmps_engine_s::MAX_LANES; ++l)
+    ok = ok && hipStreamCreateWithFlags(&e->lane_stream[l], hipStreamNonBlocking) == hipSuccess;
+  for (auto& ev : e->lane_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    e->jacobi_lanes = 1;
+  }
   *out = e;
   return 0;
 }
@@ -1409,6 +1464,13 @@ int hipmps_engine_destroy(hipmps_engine e) {
   e->hsmall.release();
   for (auto& ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : e->lane_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (int l = 1; l < hipmps_engine_s::MAX_LANES; ++l)
+    if (e->lane_stream[l]) {
+      (void)hipStreamSynchronize(e->lane_stream[l]);
+      (void)hipStreamDestroy(e->lane_stream[l]);
+    }
   (void)hipStreamDestroy(e->stream);
   delete e;
   return 0;
