@@ -624,7 +624,11 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     constexpr int ML = hipmps_engine_s::MAX_LANES;
     int nact = (int)active.size();
     JacTask* hja = (JacTask*)((char*)E->hdesc.p + desc_bytes + align_up(2 * N * sizeof(int)));
-    const int lanes = std::max(1, std::min(E->jacobi_lanes, nact));
+    // lanes only pay when one launch would already fill the chip twice over (768 resident
+    // workgroups: 256 CUs x 3 at the kernel's register count); below that they add launches
+    int total_pairs = 0;
+    for (int q : active) total_pairs += hjt[q].nb / 2;
+    const int lanes = total_pairs >= 1536 ? std::max(1, std::min(E->jacobi_lanes, nact)) : 1;
     std::vector<int> order(active);
     if (lanes > 1)
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hjt[a].nb > hjt[b].nb; });
