@@ -595,11 +595,38 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   if (n64 > 0) HIP_TRY(launch_theta(dth, n64, tile_acc[0], 0, st));
   if (N - n64 > 0) HIP_TRY(launch_theta(dth + n64, N - n64, tile_acc[1], 1, st));
   HIP_TRY(launch_jacobi_prep(dsp, djt, N, max_rpad, max_c, st));
+  // LQ lanes: with many items, contiguous slices of the LQ task arrays run their panel
+  // sequences on separate streams, so that one slice's launch tails overlap another's work
+  const int lq_lanes = nlq >= 64 ? std::max(1, std::min(E->jacobi_lanes, nlq)) : 1;
+  auto lq_slice = [&](int l, int& b, int& n) {
+    b = (int)((long long)nlq * l / lq_lanes);
+    n = (int)((long long)nlq * (l + 1) / lq_lanes) - b;
+  };
+  auto fork_lanes = [&](int lanes) -> int {
+    if (lanes < 2) return 0;
+    HIP_TRY(hipEventRecord(E->lane_ev[0], st));
+    for (int l = 1; l < lanes; ++l) HIP_TRY(hipStreamWaitEvent(E->lane_stream[l], E->lane_ev[0], 0));
+    return 0;
+  };
+  auto join_lanes = [&](int lanes) -> int {
+    for (int l = 1; l < lanes; ++l) {
+      HIP_TRY(hipEventRecord(E->lane_ev[l], E->lane_stream[l]));
+      HIP_TRY(hipStreamWaitEvent(st, E->lane_ev[l], 0));
+    }
+    return 0;
+  };
   if (nlq > 0) {
-    HIP_TRY(launch_lq(dlq, nlq, lq_max_r, st));
-    HIP_TRY(launch_lower_to_upper_h(dtr, nlq, lq_max_r, lq_max_r, st));
-    HIP_TRY(launch_lq(dlq + N, nlq, lq_max_r, st));
-    HIP_TRY(launch_lower_to_upper_h(dtr + N, nlq, lq_max_r, lq_max_rpad, st));
+    RC_TRY(fork_lanes(lq_lanes));
+    for (int l = 0; l < lq_lanes; ++l) {
+      int b, n;
+      lq_slice(l, b, n);
+      const hipStream_t ls = E->lane_stream[l];
+      HIP_TRY(launch_lq(dlq + b, n, lq_max_r, ls));
+      HIP_TRY(launch_lower_to_upper_h(dtr + b, n, lq_max_r, lq_max_r, ls));
+      HIP_TRY(launch_lq(dlq + N + b, n, lq_max_r, ls));
+      HIP_TRY(launch_lower_to_upper_h(dtr + N + b, n, lq_max_r, lq_max_rpad, ls));
+    }
+    RC_TRY(join_lanes(lq_lanes));
     for (int q = 0; q < N; ++q) {
       auto& it = items[q];
       if (!it.lq) continue;
@@ -700,8 +727,15 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   }
   if (!active.empty()) failed_conv = true;
   if (nlq > 0) {  // rows of the Jacobi result back to the column space of X
-    HIP_TRY(launch_widen(dtr + 2 * N, nlq, lq_max_c, lq_max_rpad, st));
-    HIP_TRY(launch_lq_apply_q(dlq, nlq, lq_max_r, lq_max_rpad, st));
+    RC_TRY(fork_lanes(lq_lanes));
+    for (int l = 0; l < lq_lanes; ++l) {
+      int b, n;
+      lq_slice(l, b, n);
+      const hipStream_t ls = E->lane_stream[l];
+      HIP_TRY(launch_widen(dtr + 2 * N + b, n, lq_max_c, lq_max_rpad, ls));
+      HIP_TRY(launch_lq_apply_q(dlq + b, n, lq_max_r, lq_max_rpad, ls));
+    }
+    RC_TRY(join_lanes(lq_lanes));
   }
   // restore the full JacTask array (not needed further, kept for clarity of ownership)
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[2], st));
