@@ -164,6 +164,11 @@ int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* i
 #define HIPMPS_MAX_KRAUS_OPS 16
 int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats);
 int hipmps_kraus_dims(hipmps_state s, int* out_n); /* kap_k per site */
+/* Reduced density matrix of k qubits (1 <= k <= 8), the other qubits traced out; replaces the
+ * "density_matrix" contraction of ExaTnPmpsVisitor::finalize (ExaTnPmpsVisitor.cpp:657-757).
+ * out[2 (x + 2^k x')] = Re <x|rho|x'>, + 1 the imaginary part, x = sum_i b_i 2^i over
+ * qubits[i]: the ket index runs fastest, as in the reference's flattened tensor.  cap >= 4^k. */
+int hipmps_reduced_density_matrix(hipmps_state s, const int* qubits, int k, double* out, size_t cap);
 
 /* ---- many circuits advanced together (no counterpart in the reference) ------------- */
 typedef struct {

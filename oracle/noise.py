@@ -105,6 +105,17 @@ class DensityMatrix:
     def trace(self):
         return float(np.real(np.trace(self.rho)))
 
+    def reduced(self, qubits):
+        """rho_red[x, x'] over `qubits` (x = sum_i b_i 2^i), the other qubits traced out."""
+        n, k = self.n, len(qubits)
+        t = self.rho.reshape((2,) * (2 * n))
+        keep = [_axis(n, q) for q in qubits]
+        rest = [a for a in range(n) if a not in keep]
+        # ket axes ordered qubits[k-1] .. qubits[0] (most significant first), bra the same
+        order = list(reversed(keep)) + rest + [n + a for a in reversed(keep)] + [n + a for a in rest]
+        t = np.transpose(t, order).reshape(1 << k, 1 << (n - k), 1 << k, 1 << (n - k))
+        return np.einsum("arbr->ab", t)
+
 
 class PurifiedMps:
     def __init__(self, n: int, partial_norm: str = "squared", kraus_cap: bool = True):

@@ -482,6 +482,18 @@ CASE(pmps_check_multiple_qubits_and_bell, true) {
   pmps(nullptr)->execute(b2c, g);
   auto d = (*b2c)["density-matrix-diagonal"].as<std::vector<double>>();
   EXPECT_NEAR(d[3], 1.0, 1e-12);
+  // and the density matrix is |11><11| (flattened, ket index fastest)
+  auto rho = (*b2c)["density_matrix"].as<std::vector<std::pair<double, double>>>();
+  EXPECT_EQ(rho.size(), (size_t)16);
+  EXPECT_NEAR(rho[3 + 4 * 3].first, 1.0, 1e-12);
+  // a Bell pair: <00|rho|11> = 1/2
+  auto h = compile1(R"(__qpu__ void bell2(qbit q) { H(q[0]); CX(q[0], q[1]); Measure(q[0]); Measure(q[1]); })");
+  auto bb = std::make_shared<AcceleratorBuffer>(2);
+  pmps(nullptr)->execute(bb, h);
+  auto rb = (*bb)["density_matrix"].as<std::vector<std::pair<double, double>>>();
+  EXPECT_NEAR(rb[0 + 4 * 3].first, 0.5, 1e-12);
+  EXPECT_NEAR(rb[3 + 4 * 0].first, 0.5, 1e-12);
+  EXPECT_NEAR(rb[1 + 4 * 1].first, 0.0, 1e-12);
 }
 
 // :91-115: a 100-qubit GHZ chain, 6 of its qubits measured

@@ -168,3 +168,23 @@ def test_purified_state_limits(eng):
     assert np.abs(c.expval_z_all() - s.expval_z_all()).max() < 1e-14
     c.close()
     s.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-5)])
+def test_reduced_density_matrix(eng, precision, tol):
+    say("reduced density matrix", precision)
+    n = 5
+    ops = noisy_ops(n, 4, 53, RANK2)
+    dm = run_oracle(DensityMatrix(n), ops)
+    s = run_device(eng, n, ops, precision)
+    for qs in ([2], [0, 4], [3, 1, 0]):
+        assert np.abs(s.reduced_density_matrix(qs) - dm.reduced(qs)).max() < tol
+    s.close()
+    # a pure state (no channel): the same contraction with every Kraus leg of extent 1
+    gates = C.u3_cnot_line(6, 4, seed=3)
+    p = mps.MpsState(eng, 6, precision="fp64").run(gates)
+    ref = DensityMatrix(6)
+    for g in gates:
+        ref.gate(mps.matrix_of(g), g["qubits"])
+    assert np.abs(p.reduced_density_matrix([5, 2, 0]) - ref.reduced([5, 2, 0])).max() < 1e-10
+    p.close()

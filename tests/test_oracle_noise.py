@@ -116,3 +116,19 @@ def test_kraus_cap_drops_weight_on_a_bell_pair():
     full = PurifiedMps(2, kraus_cap=False).gate(mps.gate_matrix("H"), (0,)).gate(mps.gate_matrix("CNOT"), (0, 1))
     full.channel(0, N.depolarizing(p))
     assert abs(np.real(np.trace(full.density_matrix())) - 1) < 1e-12
+
+
+def test_reduced_density_matrix():
+    n = 4
+    ops = _noisy(n, 3, 7, [N.amplitude_damping(0.2), N.phase_damping(0.1)])
+    dm = _run(DensityMatrix(n), ops)
+    assert np.abs(dm.reduced(list(range(n))) - dm.rho).max() < 1e-15  # all qubits, in order: rho
+    for qs in ([1], [3, 0], [2, 0, 1]):
+        r = dm.reduced(qs)
+        assert abs(np.trace(r) - dm.trace()) < 1e-13
+        assert np.abs(np.real(np.diag(r)) - dm.probabilities(qs)).max() < 1e-14
+        assert np.abs(r - r.conj().T).max() < 1e-14
+    # a product state: the reduced matrix of one qubit is its own density matrix
+    p = DensityMatrix(3).gate(mps.gate_matrix("H"), (1,)).gate(mps.gate_matrix("X"), (2,))
+    assert np.abs(p.reduced([1]) - 0.5 * np.ones((2, 2))).max() < 1e-15
+    assert np.abs(p.reduced([2]) - np.diag([0.0, 1.0])).max() < 1e-15

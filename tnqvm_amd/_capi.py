@@ -36,6 +36,7 @@ EXPORTS = [
     "hipmps_set_site", "hipmps_run_batch", "hipmps_serve_submit", "hipmps_serve_step",
     "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing", "hipmps_engine_set_lq_precondition",
     "hipmps_expval_diag", "hipmps_state_clone", "hipmps_apply_kraus", "hipmps_kraus_dims",
+    "hipmps_reduced_density_matrix",
 ]
 
 
@@ -140,6 +141,7 @@ def lib():
         "hipmps_state_clone": ([vp, ctypes.POINTER(vp)], ip),
         "hipmps_apply_kraus": ([vp, ip, ip, dp], ip),
         "hipmps_kraus_dims": ([vp, pint], ip),
+        "hipmps_reduced_density_matrix": ([vp, pint, ip, dp, ctypes.c_size_t], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -1060,6 +1060,7 @@ struct EnvJob {
   int z;              // 1: apply diag(d0, d1) on the physical index of this site
   double d0 = 1.0, d1 = -1.0;
   int kap = 1;        // Kraus leg of the site, traced with the physical index
+  const void* Abra = nullptr;  // bra-side site when it differs from A (projected copies)
 };
 
 // left step: E'(cr x cr) = A^H D (E A), D = diag(+-1) on the physical index when z
@@ -1083,7 +1084,7 @@ int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
     a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
     GemmTask& b = h[N + q];
     memset(&b, 0, sizeof(b));
-    b.A = j.A; b.lda = j.cr;
+    b.A = j.Abra ? j.Abra : j.A; b.lda = j.cr;
     b.B = j.X; b.ldb = j.cr;
     b.C = j.Eout; b.ldc = j.cr;
     b.M = j.cr; b.N = j.cr; b.K = d * j.cl;
@@ -1375,6 +1376,100 @@ int measure_diag(hipmps_state_s* s, const std::vector<double>& diag, double* out
   double2 v;
   RC_TRY(read_scalar(E, (char*)E->env.p + L.offL[s->n], &v));
   *out = v.x;
+  return 0;
+}
+
+// Reduced density matrix of `qubits` (the trace over the other qubits that the reference's
+// finalize contracts, ExaTnPmpsVisitor.cpp:684-757): out[x + 2^k x'] = <x| rho |x'>, x = sum_i
+// b_i 2^i over qubits[i] (the ket index fastest, as the reference's flattened tensor).
+// Left-to-right environments, one per (x, x') prefix: an unmeasured site is traced (ket and
+// bra see the same site), a measured one splits every environment into four, with the site's
+// component b on the ket side and b' on the bra side (copies through k_apply_1q with |0><b|).
+// Pure and purified states alike.
+constexpr int RDM_MAX_QUBITS = 8;
+
+int measure_rdm(hipmps_state_s* s, const std::vector<int>& qubits, std::vector<double2>& out) {
+  hipmps_engine E = s->e;
+  const hipStream_t st = E->stream;
+  const int n = s->n, k = (int)qubits.size();
+  if (k < 1 || k > RDM_MAX_QUBITS) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix of 1 to 8 qubits");
+  std::vector<int> pos(n, -1);
+  for (int i = 0; i < k; ++i) {
+    const int q = qubits[i];
+    if (q < 0 || q >= n || pos[q] >= 0) return fail(HIPMPS_E_ARG, "qubits out of range or repeated");
+    pos[q] = i;
+  }
+  int cmax = 1;
+  for (int b = 0; b <= n; ++b) cmax = std::max(cmax, s->chi[b]);
+  const size_t nenv = (size_t)1 << (2 * k);
+  const size_t mat = align_up((size_t)cmax * cmax * 16);
+  const size_t xmat = align_up((size_t)cmax * 2 * max_kap(s) * cmax * 16);
+  constexpr size_t CH = 256;  // environments per batched step (bounds the X scratch)
+  std::vector<size_t> proj_off(n, 0);
+  size_t proj_bytes = 0;
+  for (int q = 0; q < n; ++q)
+    if (pos[q] >= 0) {
+      proj_off[q] = proj_bytes;
+      proj_bytes += 2 * align_up(site_elems(s, q) * s->es);
+    }
+  const size_t total = 2 * nenv * mat + CH * xmat + proj_bytes + 4096;
+  if (total > ((size_t)32 << 30)) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix needs more than 32 GiB");
+  RC_TRY(E->env.ensure(total));
+  char* base = (char*)E->env.p;
+  char* env_a = base;
+  char* env_b = base + nenv * mat;
+  char* xs = env_b + nenv * mat;
+  char* pj = xs + CH * xmat;
+  auto proj = [&](int q, int b) { return (void*)(pj + proj_off[q] + b * align_up(site_elems(s, q) * s->es)); };
+  for (int q = 0; q < n; ++q) {
+    if (pos[q] < 0) continue;
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(hipMemcpyAsync(proj(q, b), s->sites[q].p, site_elems(s, q) * s->es, hipMemcpyDeviceToDevice, st));
+      // |0><b|: component b moved to slot 0, slot 1 zero.  The transfer step sums the ket and
+      // bra physical index together, so the ket copy of b and the bra copy of b' meet in slot 0
+      const double2 m[4] = {make_double2(b == 0 ? 1.0 : 0.0, 0), make_double2(b == 1 ? 1.0 : 0.0, 0),
+                            make_double2(0, 0), make_double2(0, 0)};
+      HIP_TRY(launch_apply_1q(proj(q, b), s->fp32, s->chi[q], s->kap[q] * s->chi[q + 1], m, st));
+    }
+  }
+  HIP_TRY(launch_fill_identity1((double2*)env_a, st));
+  std::vector<std::pair<int, int>> label(1, {0, 0}), next;
+  for (int q = 0; q < n; ++q) {
+    std::vector<EnvJob> jobs;
+    next.clear();
+    for (size_t j = 0; j < label.size(); ++j) {
+      const double2* ein = (const double2*)(env_a + j * mat);
+      if (pos[q] < 0) {
+        EnvJob job{s->sites[q].p, s->chi[q], s->chi[q + 1], ein, nullptr, (double2*)(env_b + next.size() * mat), 0};
+        job.kap = s->kap[q];
+        jobs.push_back(job);
+        next.push_back(label[j]);
+      } else {
+        for (int b = 0; b < 2; ++b)
+          for (int bp = 0; bp < 2; ++bp) {
+            EnvJob job{proj(q, b), s->chi[q], s->chi[q + 1], ein, nullptr, (double2*)(env_b + next.size() * mat), 0};
+            job.kap = s->kap[q];
+            job.Abra = proj(q, bp);
+            jobs.push_back(job);
+            next.push_back({label[j].first | (b << pos[q]), label[j].second | (bp << pos[q])});
+          }
+      }
+    }
+    for (size_t c0 = 0; c0 < jobs.size(); c0 += CH) {
+      std::vector<EnvJob> chunk(jobs.begin() + c0, jobs.begin() + std::min(jobs.size(), c0 + CH));
+      for (size_t t = 0; t < chunk.size(); ++t) chunk[t].X = (double2*)(xs + t * xmat);
+      RC_TRY(env_left(E, chunk, s->fp32));
+    }
+    std::swap(env_a, env_b);
+    label.swap(next);
+  }
+  std::vector<double2> vals(label.size());
+  HIP_TRY(hipMemcpy2DAsync(vals.data(), sizeof(double2), env_a, mat, sizeof(double2), label.size(),
+                           hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const size_t dim = (size_t)1 << k;
+  out.assign(dim * dim, make_double2(0, 0));
+  for (size_t j = 0; j < label.size(); ++j) out[label[j].first + dim * label[j].second] = vals[j];
   return 0;
 }
 
@@ -1679,6 +1774,19 @@ int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out) {
   RC_TRY(state_flush(s));
   if (k == 0) return 0;
   return measure_zz(s, pairs, k, out);
+}
+
+int hipmps_reduced_density_matrix(hipmps_state s, const int* qubits, int k, double* out, size_t cap) {
+  if (!s || !qubits || !out || k < 1) return fail(HIPMPS_E_ARG, "bad arguments");
+  if (k <= 30 && cap < ((size_t)1 << (2 * k))) return fail(HIPMPS_E_ARG, "output buffer too small");
+  RC_TRY(state_flush(s));
+  std::vector<double2> v;
+  RC_TRY(measure_rdm(s, std::vector<int>(qubits, qubits + k), v));
+  for (size_t t = 0; t < v.size(); ++t) {
+    out[2 * t] = v[t].x;
+    out[2 * t + 1] = v[t].y;
+  }
+  return 0;
 }
 
 int hipmps_expval_diag(hipmps_state s, const double* diag, double* out) {

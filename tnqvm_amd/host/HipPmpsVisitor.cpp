@@ -1,6 +1,7 @@
 // HipPmpsVisitor.cpp -- "hip-pmps" visitor over libhipmps.
 #include "HipPmpsVisitor.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -91,6 +92,17 @@ void HipPmpsVisitor::finalize() {
     ez += parity ? -probs[x] : probs[x];
   }
   buffer->addExtraInfo("density-matrix-diagonal", probs);
+  std::vector<int> sorted(m_measured);
+  std::sort(sorted.begin(), sorted.end());
+  const bool distinct = std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end();
+  if (k <= 8 && distinct) {  // the reduced density matrix, flattened ket index fastest (:684-757)
+    const size_t dim = (size_t)1 << k;
+    std::vector<double> rho(2 * dim * dim);
+    check(hipmps_reduced_density_matrix(m_state, m_measured.data(), k, rho.data(), dim * dim), "density matrix");
+    std::vector<std::pair<double, double>> flat(dim * dim);
+    for (size_t t = 0; t < flat.size(); ++t) flat[t] = {rho[2 * t], rho[2 * t + 1]};
+    buffer->addExtraInfo("density_matrix", flat);
+  }
   buffer->addExtraInfo("exp-val-z", ez);
   double tot = 0.0;
   for (double p : probs) tot += std::max(p, 0.0);

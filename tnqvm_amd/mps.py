@@ -330,6 +330,15 @@ class MpsState:
             out[x] = self.expval_diag(d)
         return out
 
+    def reduced_density_matrix(self, qubits) -> np.ndarray:
+        """rho[x, x'] = <x|rho_red|x'> over `qubits` (x = sum_i b_i 2^i), others traced out."""
+        q = np.ascontiguousarray(np.asarray(qubits, dtype=np.int32))
+        dim = 1 << len(q)
+        flat = np.zeros(dim * dim, dtype=np.complex128)
+        check(lib().hipmps_reduced_density_matrix(self.h, q.ctypes.data_as(_ip), len(q),
+                                                  flat.view(np.float64).ctypes.data_as(_dp), flat.size))
+        return flat.reshape(dim, dim).T  # flat[x + dim x'] -> rho[x, x']
+
     def kraus_dims(self) -> np.ndarray:
         out = np.zeros(self.n, dtype=np.int32)
         check(lib().hipmps_kraus_dims(self.h, out.ctypes.data_as(_ip)))
