@@ -251,10 +251,11 @@ def _graded(m, n, seed, decades=12.0):
     return (u * s) @ v.conj().T, s
 
 
-@pytest.mark.parametrize("m,n", [(1024, 1024), (512, 1024), (1024, 512)])
+@pytest.mark.parametrize("m,n", [(1024, 1024), (512, 1024), (1024, 512), (2048, 2048)])
 def test_svd_large_graded(eng, m, n):
-    """Bond dimension 512: theta' up to 1024 x 1024 through the LQ-preconditioned Jacobi, with
-    singular values spread over 12 decades (known exactly by construction)."""
+    """Bond dimension 512 (theta' up to 1024 x 1024) and 1024 (2048 x 2048, the chi_max of the
+    6x6 grid circuits) through the LQ-preconditioned Jacobi, with singular values spread over
+    12 decades (known exactly by construction)."""
     say("svd large", m, n)
     a, sref = _graded(m, n, seed=m + 3 * n)
     cl, cr = m // 2, n // 2
@@ -272,3 +273,47 @@ def test_svd_large_graded(eng, m, n):
     assert np.abs(sig - sref).max() < 1e-12
     assert np.abs(A1 @ A2 - a).max() < 1e-12
     s.close()
+
+
+def test_empty_and_one_qubit_circuits(eng):
+    """Edge cases of the request path: a circuit with no gates, a one-qubit register, both alone,
+    in a batch beside a regular circuit, and through the serving loop."""
+    say("empty and one-qubit circuits")
+    s = mps.MpsState(eng, 5)
+    assert np.abs(s.expval_z_all() - 1.0).max() < 1e-15
+    assert abs(s.norm() - 1.0) < 1e-15
+    assert abs(s.amplitudes(np.zeros((1, 5), dtype=np.int8))[0] - 1.0) < 1e-15
+    s.close()
+    one = [{"name": "H", "qubits": (0,)}, {"name": "Rz", "qubits": (0,), "params": (0.3,)},
+           {"name": "Ry", "qubits": (0,), "params": (1.1,)}]
+    ref1 = O.StateVector(1).run(one)
+    s1 = mps.MpsState(eng, 1).run(one)
+    assert abs(s1.expval_z_all()[0] - ref1.expval_z_parity([0])) < 1e-14
+    assert np.abs(s1.state_vector() - ref1.psi).max() < 1e-14
+    s1.close()
+    line = Cc.u3_cnot_line(4, 3, seed=9)
+    ref4 = O.StateVector(4).run(line)
+    jobs = [{"n": 3, "gates": [], "trunc": mps.trunc_opts(), "zz": [(0, 1)]},
+            {"n": 1, "gates": one, "trunc": mps.trunc_opts(), "zz": []},
+            {"n": 4, "gates": line, "trunc": mps.trunc_opts(), "zz": [(1, 2)]}]
+    res = eng.run_batch(jobs)
+    assert np.abs(res[0]["expval_z"] - 1.0).max() < 1e-15 and abs(res[0]["expval_zz"][0] - 1.0) < 1e-15
+    assert abs(res[1]["expval_z"][0] - ref1.expval_z_parity([0])) < 1e-14
+    for q in range(4):
+        assert abs(res[2]["expval_z"][q] - ref4.expval_z_parity([q])) < 1e-12
+    assert abs(res[2]["expval_zz"][0] - ref4.expval_z_parity([1, 2])) < 1e-12
+    srv = mps.Server(eng)
+    t0, t1 = srv.submit(3, []), srv.submit(1, one)
+    got = {}
+    for _ in range(10):
+        srv.step()
+        for t in (t0, t1):
+            if t not in got:
+                r = srv.poll(t)
+                if r is not None:
+                    got[t] = r
+        if len(got) == 2:
+            break
+    assert len(got) == 2
+    assert np.abs(got[t0]["expval_z"] - 1.0).max() < 1e-15
+    assert abs(got[t1]["expval_z"][0] - ref1.expval_z_parity([0])) < 1e-14
