@@ -159,6 +159,11 @@ struct hipmps_engine_s {
   hipStream_t lane_stream[MAX_LANES] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t lane_ev[MAX_LANES + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int jacobi_lanes = 2;
+  // mixed path: the first sweeps of SVDs with min(m, n) >= 256 form the Gram with FP32 MFMA;
+  // such a sweep never counts as the converged one, so every SVD ends with FP64 sweeps.  The
+  // FP64 path never does: rotation angles good to ~1e-7 mix large rows into small ones, which
+  // costs one-sided Jacobi its relative accuracy on graded spectra (1e-12 absolute at 2048)
+  int gram32_sweeps = 5;
   size_t budget = 0;
   size_t used = 0;  // bytes of site storage held by states
   int max_sweeps = 40;
@@ -510,6 +515,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     J.counter = d_counts + 2 * q;
     J.visited = d_counts + 2 * q + 1;
     J.tol = it.fp32 ? E->tol_mixed : E->tol;
+    J.gram32 = 0;
+    J.pad = 0;
 
     max_r = std::max(max_r, it.r);
     max_rpad = std::max(max_rpad, it.r_pad);
@@ -676,6 +683,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
         lane_begin[l] = a;
         for (int q : lane_items[l]) {
           hja[a] = hjt[q];
+          hja[a].gram32 = (sweep < E->gram32_sweeps && items[q].fp32 && hjt[q].r >= 256) ? 1 : 0;
           hja[a].pair_begin = lane_pairs[l];
           lane_pairs[l] += hja[a].nb / 2;
           lane_rounds[l] = std::max(lane_rounds[l], hja[a].nb - 1);
@@ -720,7 +728,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       E->prof.jacobi_pairs_visited += vis;
       const double c = items[q].lq ? items[q].r : items[q].c;  // columns the Jacobi works on
       E->prof.jacobi_flops += 8.0 * JP * JP * c * (double)(vis + rot);
-      if (rot == 0) sweeps_used[q] = sweep + 1;
+      const bool g32 = sweep < E->gram32_sweeps && items[q].fp32 && hjt[q].r >= 256;
+      if (rot == 0 && !g32) sweeps_used[q] = sweep + 1;
       else next.push_back(q);
     }
     active.swap(next);
@@ -1562,6 +1571,7 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
   if (opts && opts->jacobi_tol > 0) e->tol = opts->jacobi_tol;
   if (opts && opts->jacobi_tol_mixed > 0) e->tol_mixed = opts->jacobi_tol_mixed;
   for (auto& ev : e->ev) (void)hipEventCreate(&ev);
+  if (const char* v = getenv("HIPMPS_GRAM32_SWEEPS")) e->gram32_sweeps = std::max(0, atoi(v));
   if (const char* v = getenv("HIPMPS_JACOBI_LANES"))
     e->jacobi_lanes = std::max(1, std::min(hipmps_engine_s::MAX_LANES, atoi(v)));
   e->lane_stream[0] = e->stream;

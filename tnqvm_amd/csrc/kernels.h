@@ -40,6 +40,8 @@ struct JacTask {
   int* counter;      // rotated block pairs in the current sweep
   int* visited;      // block pairs visited in the current sweep
   double tol;        // relative off-diagonal below which a block pair counts as orthogonal
+  int gram32;        // this sweep forms the Gram with FP32 MFMA (rotations and updates stay FP64)
+  int pad;
 };
 
 // Everything needed after convergence: singular values, truncation, factors.

@@ -313,22 +313,25 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   const double2* rb = Y + (size_t)(bb * JB + l16) * ld;  // pair row 16 + l16 (block b)
 
   // ---- Gram tiles (0,0), (0,1), (1,1) over this wave's 4-column units u = wave, wave + 4, ... ----
-  mfma_d4 a00r = {0, 0, 0, 0}, a00i = {0, 0, 0, 0}, a01r = {0, 0, 0, 0}, a01i = {0, 0, 0, 0};
-  mfma_d4 a11r = {0, 0, 0, 0}, a11i = {0, 0, 0, 0};
-  {
-    const int nu = (C + 3) >> 2;
-    constexpr int PD = 2;
+  // FP64 MFMA, or in the early sweeps of large SVDs FP32 MFMA at twice the rate: the rotations
+  // they give are still applied in FP64 and are unitary, so the rows only lose the accuracy of
+  // the rotation angles, which the FP64 sweeps that end every SVD make up for
+  const int nu = (C + 3) >> 2;
+  constexpr int PD = 2;
+  auto ld_unit = [&](int u, double2& v0, double2& v1) {
+    const int col = 4 * u + k4;
+    if (u < nu && col < C) {
+      v0 = ra[col];
+      v1 = rb[col];
+    } else {
+      v0 = make_double2(0, 0);
+      v1 = make_double2(0, 0);
+    }
+  };
+  if (!T.gram32) {
+    mfma_d4 a00r = {0, 0, 0, 0}, a00i = {0, 0, 0, 0}, a01r = {0, 0, 0, 0}, a01i = {0, 0, 0, 0};
+    mfma_d4 a11r = {0, 0, 0, 0}, a11i = {0, 0, 0, 0};
     double2 x0[PD], x1[PD];
-    auto ld_unit = [&](int u, double2& v0, double2& v1) {
-      const int col = 4 * u + k4;
-      if (u < nu && col < C) {
-        v0 = ra[col];
-        v1 = rb[col];
-      } else {
-        v0 = make_double2(0, 0);
-        v1 = make_double2(0, 0);
-      }
-    };
 #pragma unroll
     for (int d = 0; d < PD; ++d) ld_unit(wave + 4 * d, x0[d], x1[d]);
     for (int u0 = wave; u0 < nu; u0 += 4 * PD) {
@@ -350,26 +353,75 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
         a11i = mfma64(-v1.y, v1.x, a11i);
       }
     }
-  }
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
+    // f64 D[i][j]: lane j + 16 (i % 4), register i / 4
+    for (int w = 0; w < 4; ++w) {
+      if (wave == w) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 4 * v + k4, j = l16;
-        double2 g00 = make_double2(a00r[v], a00i[v]);
-        double2 g01 = make_double2(a01r[v], a01i[v]);
-        double2 g11 = make_double2(a11r[v], a11i[v]);
-        if (w) {
-          g00 = cadd(g00, sG[i][j]);
-          g01 = cadd(g01, sG[i][JB + j]);
-          g11 = cadd(g11, sG[JB + i][JB + j]);
+        for (int v = 0; v < 4; ++v) {
+          const int i = 4 * v + k4, j = l16;
+          double2 g00 = make_double2(a00r[v], a00i[v]);
+          double2 g01 = make_double2(a01r[v], a01i[v]);
+          double2 g11 = make_double2(a11r[v], a11i[v]);
+          if (w) {
+            g00 = cadd(g00, sG[i][j]);
+            g01 = cadd(g01, sG[i][JB + j]);
+            g11 = cadd(g11, sG[JB + i][JB + j]);
+          }
+          sG[i][j] = g00;
+          sG[i][JB + j] = g01;
+          sG[JB + i][JB + j] = g11;
         }
-        sG[i][j] = g00;
-        sG[i][JB + j] = g01;
-        sG[JB + i][JB + j] = g11;
+      }
+      __syncthreads();
+    }
+  } else {
+    typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+    mfma_f4 b00r = {0, 0, 0, 0}, b00i = {0, 0, 0, 0}, b01r = {0, 0, 0, 0}, b01i = {0, 0, 0, 0};
+    mfma_f4 b11r = {0, 0, 0, 0}, b11i = {0, 0, 0, 0};
+    double2 x0[PD], x1[PD];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) ld_unit(wave + 4 * d, x0[d], x1[d]);
+    for (int u0 = wave; u0 < nu; u0 += 4 * PD) {
+#pragma unroll
+      for (int d = 0; d < PD; ++d) {
+        const float p0 = (float)x0[d].x, q0 = (float)x0[d].y, p1 = (float)x1[d].x, q1 = (float)x1[d].y;
+        ld_unit(u0 + 4 * (d + PD), x0[d], x1[d]);
+        b00r = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, p0, b00r, 0, 0, 0);
+        b01r = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, p1, b01r, 0, 0, 0);
+        b11r = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, p1, b11r, 0, 0, 0);
+        b00i = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, q0, b00i, 0, 0, 0);
+        b01i = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, q1, b01i, 0, 0, 0);
+        b11i = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, q1, b11i, 0, 0, 0);
+        b00r = __builtin_amdgcn_mfma_f32_16x16x4f32(q0, q0, b00r, 0, 0, 0);
+        b01r = __builtin_amdgcn_mfma_f32_16x16x4f32(q0, q1, b01r, 0, 0, 0);
+        b11r = __builtin_amdgcn_mfma_f32_16x16x4f32(q1, q1, b11r, 0, 0, 0);
+        b00i = __builtin_amdgcn_mfma_f32_16x16x4f32(-q0, p0, b00i, 0, 0, 0);
+        b01i = __builtin_amdgcn_mfma_f32_16x16x4f32(-q0, p1, b01i, 0, 0, 0);
+        b11i = __builtin_amdgcn_mfma_f32_16x16x4f32(-q1, p1, b11i, 0, 0, 0);
       }
     }
-    __syncthreads();
+    // f32 D[i][j]: lane j + 16 (i / 4), register i % 4; the four waves' partial tiles are
+    // summed in FP64
+    for (int w = 0; w < 4; ++w) {
+      if (wave == w) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = 4 * k4 + v, j = l16;
+          double2 g00 = make_double2((double)b00r[v], (double)b00i[v]);
+          double2 g01 = make_double2((double)b01r[v], (double)b01i[v]);
+          double2 g11 = make_double2((double)b11r[v], (double)b11i[v]);
+          if (w) {
+            g00 = cadd(g00, sG[i][j]);
+            g01 = cadd(g01, sG[i][JB + j]);
+            g11 = cadd(g11, sG[JB + i][JB + j]);
+          }
+          sG[i][j] = g00;
+          sG[i][JB + j] = g01;
+          sG[JB + i][JB + j] = g11;
+        }
+      }
+      __syncthreads();
+    }
   }
   {
     const int i = tid >> 4, j = tid & 15;
