@@ -305,6 +305,8 @@ def main():
                 "frac": round(jac_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": "k_jacobi_round", "avg_launch_us": round(avg_launch_us, 2),
                 "timing": "union span of the rounds on two streams / launches",
+                "precision_note": "mixed path: the Gram of the first 5 sweeps of large SVDs on FP32 MFMA; rotations, "
+                                  "row updates and the final sweeps on FP64 MFMA; peak is the FP64 one",
                 "launches": prof["jacobi_launches"],
                 "flops_per_launch": prof["jacobi_flops"] / max(prof["jacobi_launches"], 1)}
     nominal_flops = 736.0 * CHI ** 3  # per apply at chi=256 (theta + gate + nominal SVD)
