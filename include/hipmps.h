@@ -223,6 +223,18 @@ int hipmps_engine_set_timing(hipmps_engine e, int enabled);
  * whose SVD is ExaTN's (ExaTnMpsVisitor.cpp:1623); the singular values and the truncation are
  * the same either way. */
 int hipmps_engine_set_lq_precondition(hipmps_engine e, int min_rank);
+/* Launch and precision tunables of the batched SVD (no counterpart in the reference).  The
+ * defaults are the choice of tnqvm_amd/autotune.py on MI355X; get_tuning reports the values
+ * in force (after HIPMPS_JACOBI_LANES / HIPMPS_GRAM32_SWEEPS from the environment). */
+typedef struct {
+  int jacobi_lanes;    /* streams that the Jacobi rounds and LQ panels of large waves are dealt to, 1..4 */
+  int lane_min_pairs;  /* block pairs per sweep from which more than one lane is used */
+  int gram32_sweeps;   /* mixed states: leading sweeps of SVDs with min(m, n) >= 256 whose Gram uses
+                          FP32 MFMA (rotations and updates stay FP64); 0 = none */
+  int lq_min_rank;     /* double-LQ preconditioning for min(m, n) >= this; 0 = off */
+} hipmps_tuning;
+int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out);
+int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t);
 
 #ifdef __cplusplus
 }

@@ -317,3 +317,32 @@ def test_empty_and_one_qubit_circuits(eng):
     assert len(got) == 2
     assert np.abs(got[t0]["expval_z"] - 1.0).max() < 1e-15
     assert abs(got[t1]["expval_z"][0] - ref1.expval_z_parity([0])) < 1e-14
+
+
+def test_tuning_roundtrip_and_results(eng):
+    """Lanes only change the schedule (identical results, bit for bit); FP32-Gram sweeps only
+    the mixed path's rounding; bad values are refused."""
+    say("tuning")
+    base = eng.tuning()
+    with pytest.raises(Exception):
+        eng.set_tuning(jacobi_lanes=0)
+    circs = [Cc.haar_brickwork(12, 8, seed=0x7E5A0B00 + k) for k in range(6)]
+
+    def run(precision):
+        jobs = [{"n": 12, "gates": c, "trunc": mps.trunc_opts(1e-10, 32, precision=precision), "zz": []}
+                for c in circs]
+        return np.array([r["expval_z"] for r in eng.run_batch(jobs)])
+
+    try:
+        eng.set_tuning(jacobi_lanes=1, lane_min_pairs=0, gram32_sweeps=0)
+        a64, amx = run("fp64"), run("mixed")
+        eng.set_tuning(jacobi_lanes=2, lane_min_pairs=0, gram32_sweeps=0)
+        assert eng.tuning()["jacobi_lanes"] == 2
+        b64 = run("fp64")
+        assert np.array_equal(a64, b64)
+        eng.set_tuning(jacobi_lanes=2, lane_min_pairs=0, gram32_sweeps=5)
+        c64, cmx = run("fp64"), run("mixed")
+        assert np.array_equal(a64, c64)  # FP64 states never take the FP32 Gram
+        assert np.abs(cmx - amx).max() < 1e-6
+    finally:
+        eng.set_tuning(**base)

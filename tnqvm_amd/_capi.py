@@ -36,7 +36,7 @@ EXPORTS = [
     "hipmps_set_site", "hipmps_run_batch", "hipmps_serve_submit", "hipmps_serve_step",
     "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing", "hipmps_engine_set_lq_precondition",
     "hipmps_expval_diag", "hipmps_state_clone", "hipmps_apply_kraus", "hipmps_kraus_dims",
-    "hipmps_reduced_density_matrix",
+    "hipmps_reduced_density_matrix", "hipmps_engine_get_tuning", "hipmps_engine_set_tuning",
 ]
 
 
@@ -60,6 +60,11 @@ class TruncOpts(ctypes.Structure):
 class Gate(ctypes.Structure):
     _fields_ = [("nq", ctypes.c_int), ("q0", ctypes.c_int), ("q1", ctypes.c_int), ("pad", ctypes.c_int),
                 ("m", ctypes.c_double * 32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("jacobi_lanes", ctypes.c_int), ("lane_min_pairs", ctypes.c_int), ("gram32_sweeps", ctypes.c_int),
+                ("lq_min_rank", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -142,6 +147,8 @@ def lib():
         "hipmps_apply_kraus": ([vp, ip, ip, dp], ip),
         "hipmps_kraus_dims": ([vp, pint], ip),
         "hipmps_reduced_density_matrix": ([vp, pint, ip, dp, ctypes.c_size_t], ip),
+        "hipmps_engine_get_tuning": ([vp, ctypes.POINTER(Tuning)], ip),
+        "hipmps_engine_set_tuning": ([vp, ctypes.POINTER(Tuning)], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -159,6 +159,7 @@ struct hipmps_engine_s {
   hipStream_t lane_stream[MAX_LANES] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t lane_ev[MAX_LANES + 1] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int jacobi_lanes = 2;
+  int lane_min_pairs = 1536;  // two generations of the 768 resident Jacobi workgroups
   // mixed path: the first sweeps of SVDs with min(m, n) >= 256 form the Gram with FP32 MFMA;
   // such a sweep never counts as the converged one, so every SVD ends with FP64 sweeps.  The
   // FP64 path never does: rotation angles good to ~1e-7 mix large rows into small ones, which
@@ -662,7 +663,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     // workgroups: 256 CUs x 3 at the kernel's register count); below that they add launches
     int total_pairs = 0;
     for (int q : active) total_pairs += hjt[q].nb / 2;
-    const int lanes = total_pairs >= 1536 ? std::max(1, std::min(E->jacobi_lanes, nact)) : 1;
+    const int lanes = total_pairs >= E->lane_min_pairs ? std::max(1, std::min(E->jacobi_lanes, nact)) : 1;
     std::vector<int> order(active);
     if (lanes > 1)
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hjt[a].nb > hjt[b].nb; });
@@ -1623,6 +1624,30 @@ int hipmps_engine_set_lq_precondition(hipmps_engine e, int min_rank) {
   if (!e) return fail(HIPMPS_E_ARG, "null engine");
   if (min_rank < 0) return fail(HIPMPS_E_ARG, "min_rank must be >= 0");
   e->lq_min_rank = min_rank;
+  return 0;
+}
+
+int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out) {
+  if (!e || !out) return fail(HIPMPS_E_ARG, "null argument");
+  out->jacobi_lanes = e->jacobi_lanes;
+  out->lane_min_pairs = e->lane_min_pairs;
+  out->gram32_sweeps = e->gram32_sweeps;
+  out->lq_min_rank = e->lq_min_rank;
+  return 0;
+}
+
+int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
+  if (!e || !t) return fail(HIPMPS_E_ARG, "null argument");
+  if (t->jacobi_lanes < 1 || t->jacobi_lanes > hipmps_engine_s::MAX_LANES || t->lane_min_pairs < 0 ||
+      t->gram32_sweeps < 0 || t->lq_min_rank < 0)
+    return fail(HIPMPS_E_ARG, "tuning values out of range");
+  std::lock_guard<std::mutex> lock(e->mu);
+  int made = 1;  // lanes whose stream exists
+  while (made < hipmps_engine_s::MAX_LANES && e->lane_stream[made]) ++made;
+  e->jacobi_lanes = std::min(t->jacobi_lanes, made);
+  e->lane_min_pairs = t->lane_min_pairs;
+  e->gram32_sweeps = t->gram32_sweeps;
+  e->lq_min_rank = t->lq_min_rank;
   return 0;
 }
 

@@ -102,6 +102,19 @@ class Engine:
         """Jacobi on the double-LQ factor for min(m, n) >= min_rank (0: never)."""
         check(lib().hipmps_engine_set_lq_precondition(self.h, int(min_rank)))
 
+    def tuning(self) -> dict:
+        t = C.Tuning()
+        check(lib().hipmps_engine_get_tuning(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in C.Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = C.Tuning()
+        check(lib().hipmps_engine_get_tuning(self.h, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib().hipmps_engine_set_tuning(self.h, ctypes.byref(t)))
+        return self
+
     def profile(self, reset: bool = False) -> dict:
         p = C.Profile()
         check(lib().hipmps_engine_profile(self.h, ctypes.byref(p), int(reset)))
