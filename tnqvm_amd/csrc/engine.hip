@@ -1480,6 +1480,7 @@ int measure_rdm(hipmps_state_s* s, const std::vector<int>& qubits, std::vector<d
   const size_t dim = (size_t)1 << k;
   out.assign(dim * dim, make_double2(0, 0));
   for (size_t j = 0; j < label.size(); ++j) out[label[j].first + dim * label[j].second] = vals[j];
+  if (total > ((size_t)1 << 30)) E->env.release();  // do not hold GiBs of environments afterwards
   return 0;
 }
 
