@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--serve-grid", type=int, default=1, help="6x6 grid circuits in the serving stream")
     ap.add_argument("--serve-grid-cycles", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, circuits/s")
+    ap.add_argument("--serve-big-lane", type=int, default=0,
+                    help="serving: circuits whose bond can reach this run on the background lane (0 = off)")
     ap.add_argument("--lq-min-rank", type=int, default=-1,
                     help="Jacobi on the double-LQ factor for min(m,n) >= this (0 off; default: engine default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -374,7 +376,10 @@ def main():
         arrivals -= arrivals[0]
         log(f"serving: {args.serve_circuits} brickwork + {args.serve_grid} 6x6 grid circuits, Poisson arrivals "
             f"at {args.serve_rate}/s")
+        base = eng.tuning()
+        eng.set_tuning(big_lane_min_chi=args.serve_big_lane)
         lat, span, _ = serve_batched(eng, jobs, arrivals, args.precision)
+        eng.set_tuning(**base)
         grid_lat = [float(lat[k]) for k, j in enumerate(jobs) if j["kind"] == "grid"]
         log(f"  batched {args.precision}: {len(jobs) / span:.2f} circuits/s, p50 {np.percentile(lat, 50):.0f} ms, "
             f"p99 {np.percentile(lat, 99):.0f} ms, grid {grid_lat}")
@@ -385,6 +390,7 @@ def main():
                                "map, routed, chi_max 1024), svd-cutoff 1e-10",
                    "circuits": len(jobs), "brickwork": args.serve_circuits, "grid_6x6": args.serve_grid,
                    "arrival_rate_per_s": args.serve_rate, "precision": args.precision,
+                   "big_lane_min_chi": args.serve_big_lane,
                    "circuits_per_s": round(len(jobs) / span, 3),
                    "latency_ms_p50": out["latency_ms_p50"], "latency_ms_p99": out["latency_ms_p99"],
                    "grid_latency_ms": [round(x, 1) for x in grid_lat]}

@@ -197,7 +197,9 @@ int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits,
 /* ---- serving: continuous batching (no counterpart in the reference) ------------------ */
 int hipmps_serve_submit(hipmps_engine e, const hipmps_circuit* c, hipmps_result* result,
                         uint64_t* ticket);
-int hipmps_serve_step(hipmps_engine e, int* n_done);   /* advance every in-flight circuit */
+/* advance every in-flight circuit by one wave; circuits on the background lane (see
+ * hipmps_tuning.big_lane_min_chi) advance on a worker thread of the engine meanwhile */
+int hipmps_serve_step(hipmps_engine e, int* n_done);
 int hipmps_serve_poll(hipmps_engine e, uint64_t ticket, int* done);
 int hipmps_serve_pending(hipmps_engine e, int* queued, int* running);
 
@@ -232,6 +234,9 @@ typedef struct {
   int gram32_sweeps;   /* mixed states: leading sweeps of SVDs with min(m, n) >= 256 whose Gram uses
                           FP32 MFMA (rotations and updates stay FP64); 0 = none */
   int lq_min_rank;     /* double-LQ preconditioning for min(m, n) >= this; 0 = off */
+  int big_lane_min_chi; /* serving: circuits whose bond can reach this (min(max-bond-dim, 2^(n/2)))
+                           run on a background lane of their own, concurrently with the steps; 0 = off
+                           (default).  The large circuits then finish sooner and the others later */
 } hipmps_tuning;
 int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out);
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t);

@@ -219,6 +219,47 @@ def test_serving_alone_equals_mixed(eng):
         assert np.abs(ez - a["expval_z"]).max() < 1e-13
 
 
+def test_serving_background_lane(eng):
+    """Circuits whose bond can reach big_lane_min_chi run on the background lane, concurrently
+    with the steps of the others; every result equals the circuit run alone and the oracle."""
+    say("serving background lane")
+    base = eng.tuning()
+    try:
+        eng.set_tuning(big_lane_min_chi=16)
+        srv = mps.Server(eng)
+        # n = 10 or 11 can reach 32 (big, cap 32); n = 6 reaches 8 (small); n = 9 capped at 8 (small)
+        jobs = [(10, rand_adjacent(10, 60, seed=501), 32), (6, rand_adjacent(6, 30, seed=502), 0),
+                (11, rand_adjacent(11, 70, seed=503), 32), (9, rand_adjacent(9, 40, seed=504), 8),
+                (10, rand_adjacent(10, 50, seed=505), 0)]
+        tickets = []
+        for n, gates, cap in jobs:
+            tickets.append(srv.submit(n, gates, mps.trunc_opts(cutoff=1e-12, max_bond=cap)))
+            srv.step()
+        results = {}
+        for _ in range(2000):
+            if len(results) == len(tickets):
+                break
+            srv.step()
+            for t in tickets:
+                if t not in results:
+                    r = srv.poll(t)
+                    if r is not None:
+                        results[t] = r
+        assert len(results) == len(tickets)
+        assert srv.pending() == (0, 0)
+        for (n, gates, cap), t in zip(jobs, tickets):
+            alone = mps.MpsState(eng, n, cutoff=1e-12, max_bond=cap).run(gates)
+            ez_alone = alone.expval_z_all()
+            alone.close()
+            assert np.abs(results[t]["expval_z"] - ez_alone).max() < 1e-12
+            if cap == 0:
+                ref = O.StateVector(n).run(gates)
+                for q in range(n):
+                    assert abs(results[t]["expval_z"][q] - ref.expval_z_parity([q])) < TOL64
+    finally:
+        eng.set_tuning(**base)
+
+
 @pytest.mark.parametrize("n,depth,chi", [(12, 14, 32), (14, 12, 64)])
 def test_lq_preconditioning_same_results(n, depth, chi):
     """The Jacobi on R2 of the double Householder LQ gives the same truncated MPS as the

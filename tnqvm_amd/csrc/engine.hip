@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <condition_variable>
 #include <cfloat>
 #include <chrono>
 #include <climits>
@@ -32,6 +34,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hipmps.h"
@@ -187,6 +190,20 @@ struct hipmps_engine_s {
   std::deque<PendingCircuit*> queue;
   std::vector<PendingCircuit*> running;
   std::map<uint64_t, PendingCircuit*> by_ticket;
+  // background lane of the serving loop: circuits with max-bond-dim >= big_min_chi run on an
+  // engine of their own (stream, arena, lanes), whose waves a worker thread runs back to back
+  // while the steps advance the other circuits; the worker alone touches that engine and its
+  // circuits, and hands them back through `done`.  Off by default: the large circuit then
+  // finishes sooner, but its launches slow every wave of the others (see big_lane_min_chi)
+  int big_min_chi = 0;
+  hipmps_engine_s* big = nullptr;
+  std::thread big_worker;
+  std::mutex big_mu;
+  std::condition_variable big_cv;
+  std::deque<PendingCircuit*> big_inbox;
+  bool big_stop = false;
+  int big_rc = 0;
+  std::string big_err;
 };
 
 struct hipmps_state_s {
@@ -221,6 +238,11 @@ struct hipmps_state_s {
 
 struct PendingCircuit {
   uint64_t ticket = 0;
+  // finished: the lane that runs the circuit has written its results; done: published once
+  // that lane no longer touches it (the caller may then poll and free it)
+  bool finished = false;
+  std::atomic<bool> done{false};
+  bool big = false;  // runs on the background lane
   hipmps_circuit c{};
   std::vector<hipmps_gate> gates;
   std::vector<int> zz;
@@ -228,7 +250,6 @@ struct PendingCircuit {
   hipmps_state_s* st = nullptr;
   size_t cursor = 0;
   size_t footprint = 0;
-  bool done = false;
   std::chrono::steady_clock::time_point t0;
 };
 
@@ -1591,6 +1612,14 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
 
 int hipmps_engine_destroy(hipmps_engine e) {
   if (!e) return 0;
+  if (e->big) {  // stop the background lane before its circuits are freed
+    {
+      std::lock_guard<std::mutex> lk(e->big_mu);
+      e->big_stop = true;
+    }
+    e->big_cv.notify_all();
+    if (e->big_worker.joinable()) e->big_worker.join();
+  }
   (void)hipSetDevice(e->device);
   (void)hipStreamSynchronize(e->stream);
   // every submitted circuit that was not polled yet is owned through by_ticket
@@ -1617,6 +1646,7 @@ int hipmps_engine_destroy(hipmps_engine e) {
       (void)hipStreamDestroy(e->lane_stream[l]);
     }
   (void)hipStreamDestroy(e->stream);
+  if (e->big) hipmps_engine_destroy(e->big);
   delete e;
   return 0;
 }
@@ -1634,13 +1664,14 @@ int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out) {
   out->lane_min_pairs = e->lane_min_pairs;
   out->gram32_sweeps = e->gram32_sweeps;
   out->lq_min_rank = e->lq_min_rank;
+  out->big_lane_min_chi = e->big_min_chi;
   return 0;
 }
 
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   if (!e || !t) return fail(HIPMPS_E_ARG, "null argument");
   if (t->jacobi_lanes < 1 || t->jacobi_lanes > hipmps_engine_s::MAX_LANES || t->lane_min_pairs < 0 ||
-      t->gram32_sweeps < 0 || t->lq_min_rank < 0)
+      t->gram32_sweeps < 0 || t->lq_min_rank < 0 || t->big_lane_min_chi < 0)
     return fail(HIPMPS_E_ARG, "tuning values out of range");
   std::lock_guard<std::mutex> lock(e->mu);
   int made = 1;  // lanes whose stream exists
@@ -1649,6 +1680,7 @@ int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   e->lane_min_pairs = t->lane_min_pairs;
   e->gram32_sweeps = t->gram32_sweeps;
   e->lq_min_rank = t->lq_min_rank;
+  e->big_min_chi = t->big_lane_min_chi;
   return 0;
 }
 
@@ -1967,7 +1999,7 @@ int finish_circuit(PendingCircuit* pc) {
     r->status = s->status;
     r->latency_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pc->t0).count();
   }
-  pc->done = true;
+  pc->finished = true;
   return 0;
 }
 
@@ -1975,7 +2007,7 @@ int finish_circuit(PendingCircuit* pc) {
 int advance_all(hipmps_engine E, std::vector<PendingCircuit*>& run) {
   std::vector<WaveItem> items;
   for (auto* pc : run) {
-    if (pc->done) continue;
+    if (pc->finished) continue;
     hipmps_state_s* s = pc->st;
     while (pc->cursor < pc->gates.size()) {
       const hipmps_gate& g = pc->gates[pc->cursor];
@@ -1996,7 +2028,7 @@ int advance_all(hipmps_engine E, std::vector<PendingCircuit*>& run) {
   }
   RC_TRY(run_wave(E, items));
   for (auto* pc : run) {
-    if (pc->done) continue;
+    if (pc->finished) continue;
     hipmps_state_s* s = pc->st;
     s->wave.clear();
     std::fill(s->busy.begin(), s->busy.end(), 0);
@@ -2010,6 +2042,81 @@ int start_circuit(hipmps_engine E, PendingCircuit* pc) {
   RC_TRY(hipmps_state_create(E, pc->c.n_qubits, &pc->c.trunc, &st));
   pc->st = st;
   return 0;
+}
+
+// The background lane's worker: take newly admitted circuits, advance everything it owns by
+// one wave, free the finished ones and publish them (done), and repeat while it has work.
+void big_lane_loop(hipmps_engine E) {
+  hipmps_engine B = E->big;
+  (void)hipSetDevice(B->device);
+  std::vector<PendingCircuit*> run;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(E->big_mu);
+      E->big_cv.wait(lk, [&] { return E->big_stop || !E->big_inbox.empty() || !run.empty(); });
+      if (E->big_stop) return;  // circuits still owned are freed by hipmps_engine_destroy
+      while (!E->big_inbox.empty()) {
+        run.push_back(E->big_inbox.front());
+        E->big_inbox.pop_front();
+      }
+    }
+    int rc = 0;
+    for (auto* pc : run)
+      if (!pc->st && rc == 0) rc = start_circuit(B, pc);
+    if (rc == 0) rc = advance_all(B, run);
+    std::string err = rc ? hipmps_last_error() : "";
+    std::vector<PendingCircuit*> keep;
+    for (auto* pc : run) {
+      if (rc != 0 && !pc->finished) {
+        if (pc->res) pc->res->status = rc;
+        pc->finished = true;
+      }
+      if (pc->finished) {
+        if (pc->st) hipmps_state_destroy(pc->st);
+        pc->st = nullptr;
+        pc->done.store(true, std::memory_order_release);
+      } else {
+        keep.push_back(pc);
+      }
+    }
+    run.swap(keep);
+    {
+      std::lock_guard<std::mutex> lk(E->big_mu);
+      if (rc != 0 && E->big_rc == 0) {
+        E->big_rc = rc;
+        E->big_err = err;
+      }
+    }
+    E->big_cv.notify_all();
+  }
+}
+
+int ensure_big_lane(hipmps_engine E) {
+  if (E->big) return 0;
+  hipmps_engine_opts o{};
+  o.device = E->device;
+  o.hbm_budget_bytes = E->budget;
+  o.jacobi_max_sweeps = E->max_sweeps;
+  o.jacobi_tol = E->tol;
+  o.jacobi_tol_mixed = E->tol_mixed;
+  hipmps_engine B = nullptr;
+  RC_TRY(hipmps_engine_create(&o, &B));
+  hipmps_tuning t{};
+  (void)hipmps_engine_get_tuning(E, &t);
+  (void)hipmps_engine_set_tuning(B, &t);
+  B->big_min_chi = 0;
+  E->big = B;
+  E->big_stop = false;
+  E->big_worker = std::thread(big_lane_loop, E);
+  return 0;
+}
+
+// largest bond a circuit can reach: min(max-bond-dim, 2^floor(n/2))
+int projected_max_bond(const hipmps_circuit& c) {
+  const int half = c.n_qubits / 2;
+  const long long full = half >= 30 ? (1LL << 30) : (1LL << half);
+  const long long cap = c.trunc.max_bond_dim > 0 ? c.trunc.max_bond_dim : (1LL << 30);
+  return (int)std::min(full, cap);
 }
 
 }  // namespace
@@ -2035,7 +2142,7 @@ int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits, hip
   }
   while (rc == 0) {
     bool all_done = true;
-    for (auto* pc : run) all_done &= pc->done;
+    for (auto* pc : run) all_done &= pc->finished;
     if (all_done) break;
     rc = advance_all(e, run);
   }
@@ -2075,24 +2182,49 @@ int hipmps_serve_step(hipmps_engine e, int* n_done) {
   if (!e) return fail(HIPMPS_E_ARG, "null engine");
   std::lock_guard<std::mutex> lock(e->mu);
   RC_TRY(set_device(e));
-  // admission by projected HBM footprint
+  // admission by projected HBM footprint; circuits that can reach big_min_chi go to the
+  // background lane
   size_t inflight = 0;
   for (auto* pc : e->running) inflight += pc->footprint;
   while (!e->queue.empty()) {
     PendingCircuit* pc = e->queue.front();
     if (!e->running.empty() && inflight + pc->footprint > e->budget) break;
     e->queue.pop_front();
-    RC_TRY(start_circuit(e, pc));
+    pc->big = e->big_min_chi > 0 && projected_max_bond(pc->c) >= e->big_min_chi;
+    if (pc->big) {
+      RC_TRY(ensure_big_lane(e));
+      {
+        std::lock_guard<std::mutex> lk(e->big_mu);
+        e->big_inbox.push_back(pc);
+      }
+      e->big_cv.notify_all();
+    } else {
+      RC_TRY(start_circuit(e, pc));
+    }
     e->running.push_back(pc);
     inflight += pc->footprint;
   }
+  std::vector<PendingCircuit*> small;
+  bool any_big = false;
+  for (auto* pc : e->running) {
+    if (pc->big) any_big = true;
+    else small.push_back(pc);
+  }
+  if (!small.empty()) {
+    RC_TRY(advance_all(e, small));
+  } else if (any_big) {  // only background circuits in flight: wait for some progress
+    std::unique_lock<std::mutex> lk(e->big_mu);
+    e->big_cv.wait_for(lk, std::chrono::milliseconds(20));
+  }
   int done = 0;
-  if (!e->running.empty()) RC_TRY(advance_all(e, e->running));
   std::vector<PendingCircuit*> keep;
   for (auto* pc : e->running) {
-    if (pc->done) {
+    if (!pc->big && pc->finished) {
       hipmps_state_destroy(pc->st);
       pc->st = nullptr;
+      pc->done.store(true, std::memory_order_release);
+      ++done;
+    } else if (pc->big && pc->done.load(std::memory_order_acquire)) {
       ++done;
     } else {
       keep.push_back(pc);
@@ -2100,6 +2232,12 @@ int hipmps_serve_step(hipmps_engine e, int* n_done) {
   }
   e->running.swap(keep);
   if (n_done) *n_done = done;
+  std::lock_guard<std::mutex> lk(e->big_mu);
+  if (e->big_rc != 0) {
+    const int rc = e->big_rc;
+    e->big_rc = 0;
+    return fail(rc, "background lane: " + e->big_err);
+  }
   return 0;
 }
 
@@ -2108,8 +2246,13 @@ int hipmps_serve_poll(hipmps_engine e, uint64_t ticket, int* done) {
   std::lock_guard<std::mutex> lock(e->mu);
   auto it = e->by_ticket.find(ticket);
   if (it == e->by_ticket.end()) return fail(HIPMPS_E_ARG, "unknown ticket");
-  *done = it->second->done ? 1 : 0;
-  if (it->second->done) {
+  const bool d = it->second->done.load(std::memory_order_acquire);
+  *done = d ? 1 : 0;
+  if (d) {
+    // a background-lane circuit is published by its worker and may still be listed as running
+    // until the next step: drop it from that list before it is freed
+    auto r = std::find(e->running.begin(), e->running.end(), it->second);
+    if (r != e->running.end()) e->running.erase(r);
     delete it->second;
     e->by_ticket.erase(it);
   }
