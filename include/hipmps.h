@@ -164,8 +164,10 @@ int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* i
 #define HIPMPS_MAX_KRAUS_OPS 16
 int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats);
 int hipmps_kraus_dims(hipmps_state s, int* out_n); /* kap_k per site */
-/* Reduced density matrix of k qubits (1 <= k <= 8), the other qubits traced out; replaces the
- * "density_matrix" contraction of ExaTnPmpsVisitor::finalize (ExaTnPmpsVisitor.cpp:657-757).
+/* Reduced density matrix of k qubits (1 <= k <= 15, the reference's limit), the other qubits
+ * traced out; with k = n the full density matrix.  Replaces the "density_matrix" contraction of
+ * ExaTnPmpsVisitor::finalize (ExaTnPmpsVisitor.cpp:657-761).  HIPMPS_E_NOMEM when its device
+ * workspace (about 3 x 4^k x 16 bytes at the end of the sweep) does not fit.
  * out[2 (x + 2^k x')] = Re <x|rho|x'>, + 1 the imaginary part, x = sum_i b_i 2^i over
  * qubits[i]: the ket index runs fastest, as in the reference's flattened tensor.  cap >= 4^k. */
 int hipmps_reduced_density_matrix(hipmps_state s, const int* qubits, int k, double* out, size_t cap);
@@ -237,6 +239,9 @@ typedef struct {
   int big_lane_min_chi; /* serving: circuits whose bond can reach this (min(max-bond-dim, 2^(n/2)))
                            run on a background lane of their own, concurrently with the steps; 0 = off
                            (default).  The large circuits then finish sooner and the others later */
+  int rdm_block_elems;  /* density matrices: the per-site intermediate of the sweep is held a block of
+                           (p, kappa) at a time above max(this, largest environment) complex
+                           elements; default 2^24 */
 } hipmps_tuning;
 int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out);
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t);

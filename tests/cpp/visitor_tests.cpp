@@ -510,6 +510,13 @@ CASE(pmps_check_many_qubits, true) {
   auto d = (*buffer)["density-matrix-diagonal"].as<std::vector<double>>();
   EXPECT_NEAR(d[0], 0.5, 1e-10);
   EXPECT_NEAR(d[63], 0.5, 1e-10);
+  // n > 15: the density matrix of the measured qubits; a proper subset of GHZ qubits has no
+  // coherence left, diag(1/2, 0, ..., 0, 1/2)
+  auto rho = (*buffer)["density_matrix"].as<std::vector<std::pair<double, double>>>();
+  EXPECT_EQ(rho.size(), (size_t)4096);
+  EXPECT_NEAR(rho[0].first, 0.5, 1e-10);
+  EXPECT_NEAR(rho[63 + 64 * 63].first, 0.5, 1e-10);
+  EXPECT_NEAR(rho[0 + 64 * 63].first, 0.0, 1e-10);
 }
 
 // a routed gate: lnn-transform swaps are visited, and carry noise like any other gate
@@ -522,6 +529,17 @@ CASE(pmps_routed_gate_noise, true) {
   EXPECT_TRUE(d[1] < 0.95 && d[1] > 0.5);
   auto kd = (*b)["kraus-dims"].as<std::vector<int>>();
   EXPECT_TRUE(kd[3] > 1);
+  // n <= 15: the full density matrix of all 4 qubits (ExaTnPmpsVisitor.cpp:664-683); its
+  // diagonal summed over qubits 0..2 is the distribution of the measured qubit 3
+  auto rho = (*b)["density_matrix"].as<std::vector<std::pair<double, double>>>();
+  EXPECT_EQ(rho.size(), (size_t)256);
+  double p1 = 0.0, tr = 0.0;
+  for (size_t x = 0; x < 16; ++x) {
+    tr += rho[x + 16 * x].first;
+    if (x & 8) p1 += rho[x + 16 * x].first;
+  }
+  EXPECT_NEAR(p1, d[1], 1e-12);
+  EXPECT_NEAR(tr, (*b)["norm"].as<double>(), 1e-10);
 }
 
 }  // namespace

@@ -9,6 +9,8 @@
 * Noise-model runs in the shape of the reference's own tests (ExaTnPmpsTester.cpp), with the
   ibmqx2 backend JSON fixture from the reference's test resources.
 * A purified state refuses amplitudes; clone keeps the Kraus legs.
+* Reduced and full density matrices up to the reference's 15-qubit limit, against the dense
+  density matrix, the state vector and the GHZ closed form.
 """
 import os
 import sys
@@ -16,6 +18,7 @@ import sys
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from oracle.noise import DensityMatrix, PurifiedMps
 from tnqvm_amd import _capi as capi
 from tnqvm_amd import circuits as C
@@ -147,12 +150,20 @@ def test_reference_scenarios(eng):
                       measured=[0, 1], shots=1024, seed=3)
     p11 = r["counts"].get("11", 0) / 1024
     assert 0.8 < p11 < 0.99
+    rho = r["density_matrix"]  # n <= 15: the full matrix, whose diagonal is the distribution
+    assert rho.shape == (4, 4) and np.abs(np.real(np.diag(rho)) - r["probabilities"]).max() < 1e-12
     say("ghz 100")
     gates = [{"name": "H", "qubits": (0,)}] + [{"name": "CNOT", "qubits": (i, i + 1)} for i in range(99)]
     r = mps.run_noisy(eng, 100, gates, None, measured=[0, 6, 8, 36, 72, 98], shots=1024, seed=4)
     pr = r["probabilities"]
     assert abs(pr[0] - 0.5) < 1e-10 and abs(pr[-1] - 0.5) < 1e-10
     assert abs(r["counts"].get("111111", 0) / 1024 - 0.5) < 0.1
+    # n > 15: the measured qubits' matrix; of a proper subset of GHZ qubits it has no coherence
+    # (the traced qubits tell the two branches apart): diag(1/2, 0, ..., 0, 1/2)
+    rho = r["density_matrix"]
+    assert r["density_matrix_qubits"] == [0, 6, 8, 36, 72, 98] and rho.shape == (64, 64)
+    assert abs(rho[0, 0] - 0.5) < 1e-10 and abs(rho[63, 63] - 0.5) < 1e-10
+    assert abs(np.abs(rho).sum() - 1.0) < 1e-9
 
 
 def test_purified_state_limits(eng):
@@ -188,3 +199,92 @@ def test_reduced_density_matrix(eng, precision, tol):
         ref.gate(mps.matrix_of(g), g["qubits"])
     assert np.abs(p.reduced_density_matrix([5, 2, 0]) - ref.reduced([5, 2, 0])).max() < 1e-10
     p.close()
+
+
+def _sv_reduced(psi, n, qs):
+    """rho_red[x, x'] of a dense state (qubit q = bit q), x = sum_i b_i 2^i over qs."""
+    t = psi.reshape((2,) * n)
+    keep = [n - 1 - q for q in qs]
+    rest = [a for a in range(n) if a not in keep]
+    m = np.transpose(t, list(reversed(keep)) + rest).reshape(1 << len(qs), -1)
+    return m @ m.conj().T
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("mixed", 1e-5)])
+def test_full_density_matrix(eng, precision, tol):
+    """k = n: the full density matrix that the reference stores for n <= 15
+    (ExaTnPmpsVisitor.cpp:664-683), in qubit order and permuted."""
+    say("full density matrix", precision)
+    n = 6
+    ops = noisy_ops(n, 4, 61, RANK2)
+    dm = run_oracle(DensityMatrix(n), ops)
+    s = run_device(eng, n, ops, precision)
+    for qs in (list(range(n)), [4, 1, 5, 0, 3, 2]):
+        assert np.abs(s.reduced_density_matrix(qs) - dm.reduced(qs)).max() < tol
+    s.close()
+
+
+def test_density_matrix_many_qubits(eng):
+    """Up to the reference's 15-qubit limit: 10 of 12 qubits of a pure state against the state
+    vector; 12 of a 20-qubit GHZ state (4^12 labels), a proper subset whose matrix is
+    diag(1/2, 0, ..., 0, 1/2); all 13 qubits of a 13-qubit GHZ state, which keeps the coherences."""
+    say("density matrix 10 of 12")
+    n = 12
+    gates = C.u3_cnot_line(n, 5, seed=71)
+    s = mps.MpsState(eng, n, precision="fp64").run(gates)
+    sv = O.StateVector(n).run(gates)
+    qs = [11, 0, 5, 3, 7, 1, 9, 2, 10, 6]
+    assert np.abs(s.reduced_density_matrix(qs) - _sv_reduced(sv.psi, n, qs)).max() < 1e-10
+    s.close()
+    say("density matrix 12 of GHZ 20")
+    n = 20
+    g = mps.MpsState(eng, n, precision="fp64")
+    g.run([{"name": "H", "qubits": (0,)}] + [{"name": "CNOT", "qubits": (i, i + 1)} for i in range(n - 1)])
+    qs = [0, 2, 3, 5, 8, 9, 11, 13, 14, 16, 18, 19]
+    rho = g.reduced_density_matrix(qs)
+    d = 1 << len(qs)
+    assert rho.shape == (d, d)
+    assert abs(rho[0, 0] - 0.5) < 1e-10 and abs(rho[d - 1, d - 1] - 0.5) < 1e-10
+    assert abs(np.abs(rho).sum() - 1.0) < 1e-9
+    g.close()
+    say("density matrix of GHZ 13")
+    n = 13
+    g = mps.MpsState(eng, n, precision="fp64")
+    g.run([{"name": "H", "qubits": (0,)}] + [{"name": "CNOT", "qubits": (i, i + 1)} for i in range(n - 1)])
+    rho = g.reduced_density_matrix(list(range(n)))
+    d = 1 << n
+    for a, b in ((0, 0), (0, d - 1), (d - 1, 0), (d - 1, d - 1)):
+        assert abs(rho[a, b] - 0.5) < 1e-10
+    assert abs(np.abs(rho).sum() - 2.0) < 1e-9
+    g.close()
+    # beyond 15 qubits: refused before any work (a one-element buffer suffices to show it)
+    big = mps.MpsState(eng, 16)
+    q16 = np.arange(16, dtype=np.int32)
+    buf = np.zeros(2)
+    rc = capi.lib().hipmps_reduced_density_matrix(big.h, q16.ctypes.data_as(capi.ctypes.POINTER(capi.ctypes.c_int)),
+                                                  16, buf.ctypes.data_as(capi.ctypes.POINTER(capi.ctypes.c_double)), 1)
+    assert rc == capi.HIPMPS_E_UNSUPPORTED
+    big.close()
+
+
+@pytest.mark.parametrize("block", [1 << 24, 64, 1])
+def test_density_matrix_wide_kraus(eng, block):
+    """Amplitude damping after every gate: wide Kraus legs.  With a small rdm_block_elems the
+    sweep holds its per-site intermediate a block of (p, kappa) at a time and accumulates the
+    bra side over the blocks; every block size gives the dense density matrix (rank-2 channels:
+    the 1e-9 cut is the only approximation)."""
+    say("density matrix, wide Kraus legs, block", block)
+    n = 8
+    ops = noisy_ops(n, 4, 81, [N.amplitude_damping(0.05)])
+    dm = run_oracle(DensityMatrix(n), ops)
+    s = run_device(eng, n, ops, "fp64")
+    assert max(s.kraus_dims()) >= 16
+    base = eng.tuning()
+    try:
+        eng.set_tuning(rdm_block_elems=block)
+        assert np.abs(s.reduced_density_matrix(list(range(n))) - dm.reduced(list(range(n)))).max() < 1e-8
+        qs = [7, 2, 5]
+        assert np.abs(s.reduced_density_matrix(qs) - dm.reduced(qs)).max() < 1e-8
+    finally:
+        eng.set_tuning(**base)
+    s.close()

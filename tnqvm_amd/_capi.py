@@ -64,7 +64,8 @@ class Gate(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("jacobi_lanes", ctypes.c_int), ("lane_min_pairs", ctypes.c_int), ("gram32_sweeps", ctypes.c_int),
-                ("lq_min_rank", ctypes.c_int), ("big_lane_min_chi", ctypes.c_int)]
+                ("lq_min_rank", ctypes.c_int), ("big_lane_min_chi", ctypes.c_int),
+                ("rdm_block_elems", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

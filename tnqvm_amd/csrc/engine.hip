@@ -196,6 +196,9 @@ struct hipmps_engine_s {
   // circuits, and hands them back through `done`.  Off by default: the large circuit then
   // finishes sooner, but its launches slow every wave of the others (see big_lane_min_chi)
   int big_min_chi = 0;
+  // density-matrix sweep: the per-site intermediate is held a block of (p, kappa) at a time when
+  // it exceeds max(this many elements, the largest environment)
+  int rdm_block_elems = 1 << 24;
   hipmps_engine_s* big = nullptr;
   std::thread big_worker;
   std::mutex big_mu;
@@ -1411,97 +1414,306 @@ int measure_diag(hipmps_state_s* s, const std::vector<double>& diag, double* out
 }
 
 // Reduced density matrix of `qubits` (the trace over the other qubits that the reference's
-// finalize contracts, ExaTnPmpsVisitor.cpp:684-757): out[x + 2^k x'] = <x| rho |x'>, x = sum_i
-// b_i 2^i over qubits[i] (the ket index fastest, as the reference's flattened tensor).
-// Left-to-right environments, one per (x, x') prefix: an unmeasured site is traced (ket and
-// bra see the same site), a measured one splits every environment into four, with the site's
-// component b on the ket side and b' on the bra side (copies through k_apply_1q with |0><b|).
-// Pure and purified states alike.
-constexpr int RDM_MAX_QUBITS = 8;
+// finalize contracts, ExaTnPmpsVisitor.cpp:657-761; at most 15 qubits, :661): out[x + 2^k x'] =
+// <x| rho |x'>, x = sum_i b_i 2^i over qubits[i] (the ket index fastest, as the reference's
+// flattened tensor).  Pure and purified states alike.
+//
+// Two half-sweeps meet at a bond m: sites 0..m-1 from the left, sites n-1..m from the right
+// (on copies with the bonds swapped, so that both halves run the same step).  A half-sweep
+// keeps the environments of all its labels A (ket and bra components of the measured sites
+// passed so far, 4 per site) as one tensor E[l'][A][l] (bra bond, label, ket bond), so that a
+// site costs two GEMMs over all labels at once:
+//   ket side  X[s][l'][A][r] = sum_l E[l'][A][l] A[l][s][r]            one task per s = (p, kappa)
+//   bra side  E'[r'][A][r]   = sum_(s, l') conj(B[s][l'][r']) X[s][l'][A][r]
+// with B the site copied to s-major order.  An unmeasured site is traced (ket and bra are the
+// same site); at a measured one the ket and bra take the site's copies with |0><b| applied
+// (component b moved to slot 0, the other slot zero), and the four (b, b') give the labels
+// A' = (2b + b') |A| + A.  X holds a block of s at a time when all of them would not fit; the
+// bra side then accumulates over the blocks.  The halves join in one GEMM over both bonds,
+// rho[A_L][A_R] = sum_(l', l) E_L[l'][A_L][l] E_R[l'][A_R][l], and a scatter puts the labels in
+// the output order.  The split m minimises the device workspace: about 4^(k/2) chi^2 per half
+// instead of 4^k chi^2 for a single sweep.
+constexpr int RDM_MAX_QUBITS = 15;
 
-int measure_rdm(hipmps_state_s* s, const std::vector<int>& qubits, std::vector<double2>& out) {
+struct RdmSite {
+  const void* ket[2];  // [l][s][r]
+  const void* bra[2];  // [s][l][r]
+  int cl, d, cr;
+  bool meas;
+};
+
+struct RdmPlan {
+  int m = -1;
+  size_t es = 0, xs = 0, el = 0, er = 0, nl = 1, nr = 1, copies = 0, xcap = 0, total = 0;
+};
+
+constexpr size_t RDM_INT_MAX = ((size_t)1 << 31) - 1;
+// k_gemm runs one 256-thread block per tile and a dispatch holds at most 2^32 work-items
+constexpr size_t RDM_TILES_MAX = ((size_t)1 << 24) - 1;
+
+// block of s that X holds for a site with `per` = cl |A| cr elements and `tiles` ket-side GEMM
+// tiles per s
+inline int rdm_block(int d, size_t per, size_t xcap, size_t tiles) {
+  size_t w = (size_t)d * per <= xcap ? (size_t)d : std::max<size_t>(1, std::min<size_t>(d, xcap / per));
+  w = std::min(w, std::max<size_t>(1, RDM_TILES_MAX / std::max<size_t>(1, tiles)));
+  return (int)w;
+}
+
+int rdm_half(hipmps_engine E, const std::vector<RdmSite>& v, int fp32, size_t es_site, char* ea, char* eb, char* xs,
+             char* dst, size_t xcap) {
+  const hipStream_t st = E->stream;
+  HIP_TRY(launch_fill_identity1((double2*)(v.empty() ? dst : ea), st));
+  size_t nA = 1;
+  for (size_t t = 0; t < v.size(); ++t) {
+    const RdmSite& S = v[t];
+    const int cl = S.cl, d = S.d, cr = S.cr;
+    char* out = (t + 1 == v.size()) ? dst : eb;
+    const size_t per = (size_t)cl * nA * cr;
+    const int w = rdm_block(d, per, xcap, ((cl * nA + 31) / 32) * (((size_t)cr + 31) / 32));
+    const int nket = S.meas ? 2 : 1, nbra = nket;
+    const int nch = (d + w - 1) / w;
+    const size_t ndesc = (size_t)nket * (d + (size_t)nch * nbra);
+    RC_TRY(E->hdesc.ensure(ndesc * sizeof(GemmTask)));
+    RC_TRY(E->desc.ensure(ndesc * sizeof(GemmTask)));
+    GemmTask* h = (GemmTask*)E->hdesc.p;
+    memset(h, 0, ndesc * sizeof(GemmTask));
+    struct Launch {
+      size_t off;
+      int ntask, tiles, bra;
+    };
+    std::vector<Launch> launches;
+    size_t o = 0;
+    for (int b = 0; b < nket; ++b)
+      for (int c = 0; c < nch; ++c) {
+        const int s0 = c * w, ws = std::min(w, d - s0);
+        int t1 = 0;
+        for (int sl = 0; sl < ws; ++sl) {
+          GemmTask& g = h[o + sl];
+          g.A = ea;
+          g.lda = cl;
+          g.M = (int)(cl * nA);
+          g.K = cl;
+          g.B = (const char*)S.ket[b] + (size_t)(s0 + sl) * cr * es_site;
+          g.ldb = d * cr;
+          g.N = cr;
+          g.C = (double2*)xs + (size_t)sl * per;
+          g.ldc = cr;
+          g.tiles_m = (g.M + 31) / 32;
+          g.tiles_n = (g.N + 31) / 32;
+          g.tile_begin = t1;
+          t1 += g.tiles_m * g.tiles_n;
+        }
+        launches.push_back({o, ws, t1, 0});
+        o += ws;
+        int t2 = 0;
+        for (int bp = 0; bp < nbra; ++bp) {
+          GemmTask& g = h[o + bp];
+          const int slot = S.meas ? 2 * b + bp : 0;
+          g.A = (const char*)S.bra[bp] + (size_t)s0 * cl * cr * es_site;
+          g.lda = cr;
+          g.M = cr;
+          g.K = ws * cl;
+          g.B = xs;
+          g.ldb = (int)(nA * cr);
+          g.N = (int)(nA * cr);
+          g.C = (double2*)out + (size_t)slot * nA * cr;
+          g.ldc = (int)((S.meas ? 4 : 1) * nA * cr);
+          g.accumulate = s0 > 0;
+          g.tiles_m = (g.M + 31) / 32;
+          g.tiles_n = (g.N + 31) / 32;
+          g.tile_begin = t2;
+          t2 += g.tiles_m * g.tiles_n;
+        }
+        launches.push_back({o, nbra, t2, 1});
+        o += nbra;
+      }
+    GemmTask* dd = (GemmTask*)E->desc.p;
+    HIP_TRY(hipMemcpyAsync(dd, h, ndesc * sizeof(GemmTask), hipMemcpyHostToDevice, st));
+    // stream order: each block's bra side reads X before the next block's ket side rewrites it
+    for (const auto& L : launches)
+      HIP_TRY(launch_gemm(dd + L.off, L.ntask, L.tiles, L.bra ? 2 : 0, 0, L.bra ? fp32 : 0, L.bra ? 0 : fp32, 0, st));
+    HIP_TRY(hipStreamSynchronize(st));  // staging reuse
+    nA *= S.meas ? 4 : 1;
+    if (out == eb) std::swap(ea, eb);
+  }
+  return 0;
+}
+
+int measure_rdm(hipmps_state_s* s, const std::vector<int>& qubits, double* out) {
   hipmps_engine E = s->e;
   const hipStream_t st = E->stream;
   const int n = s->n, k = (int)qubits.size();
-  if (k < 1 || k > RDM_MAX_QUBITS) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix of 1 to 8 qubits");
+  if (k < 1 || k > RDM_MAX_QUBITS) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix of 1 to 15 qubits");
   std::vector<int> pos(n, -1);
   for (int i = 0; i < k; ++i) {
     const int q = qubits[i];
     if (q < 0 || q >= n || pos[q] >= 0) return fail(HIPMPS_E_ARG, "qubits out of range or repeated");
     pos[q] = i;
   }
-  int cmax = 1;
-  for (int b = 0; b <= n; ++b) cmax = std::max(cmax, s->chi[b]);
-  const size_t nenv = (size_t)1 << (2 * k);
-  const size_t mat = align_up((size_t)cmax * cmax * 16);
-  const size_t xmat = align_up((size_t)cmax * 2 * max_kap(s) * cmax * 16);
-  constexpr size_t CH = 256;  // environments per batched step (bounds the X scratch)
-  std::vector<size_t> proj_off(n, 0);
-  size_t proj_bytes = 0;
-  for (int q = 0; q < n; ++q)
-    if (pos[q] >= 0) {
-      proj_off[q] = proj_bytes;
-      proj_bytes += 2 * align_up(site_elems(s, q) * s->es);
-    }
-  const size_t total = 2 * nenv * mat + CH * xmat + proj_bytes + 4096;
-  if (total > ((size_t)32 << 30)) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix needs more than 32 GiB");
-  RC_TRY(E->env.ensure(total));
-  char* base = (char*)E->env.p;
-  char* env_a = base;
-  char* env_b = base + nenv * mat;
-  char* xs = env_b + nenv * mat;
-  char* pj = xs + CH * xmat;
-  auto proj = [&](int q, int b) { return (void*)(pj + proj_off[q] + b * align_up(site_elems(s, q) * s->es)); };
-  for (int q = 0; q < n; ++q) {
-    if (pos[q] < 0) continue;
-    for (int b = 0; b < 2; ++b) {
-      HIP_TRY(hipMemcpyAsync(proj(q, b), s->sites[q].p, site_elems(s, q) * s->es, hipMemcpyDeviceToDevice, st));
-      // |0><b|: component b moved to slot 0, slot 1 zero.  The transfer step sums the ket and
-      // bra physical index together, so the ket copy of b and the bra copy of b' meet in slot 0
-      const double2 m[4] = {make_double2(b == 0 ? 1.0 : 0.0, 0), make_double2(b == 1 ? 1.0 : 0.0, 0),
-                            make_double2(0, 0), make_double2(0, 0)};
-      HIP_TRY(launch_apply_1q(proj(q, b), s->fp32, s->chi[q], s->kap[q] * s->chi[q + 1], m, st));
-    }
+  const size_t nout = (size_t)1 << (2 * k);
+  std::vector<size_t> sb(n);
+  for (int q = 0; q < n; ++q) sb[q] = align_up(site_elems(s, q) * s->es);
+  // the workspace of a split at bond m; false when an int argument would overflow
+  auto plan = [&](int m, RdmPlan& P) -> bool {
+    P = RdmPlan{};
+    P.m = m;
+    size_t max_e = 1;
+    auto pass_e = [&](int q0, int q1, int step, size_t& labels) {
+      size_t nA = 1;
+      for (int q = q0; q != q1; q += step) {
+        const size_t cl = step > 0 ? s->chi[q] : s->chi[q + 1];
+        max_e = std::max(max_e, nA * cl * cl);
+        nA *= pos[q] >= 0 ? 4 : 1;
+      }
+      labels = nA;
+    };
+    pass_e(0, m, 1, P.nl);
+    pass_e(n - 1, m - 1, -1, P.nr);
+    const size_t cm = s->chi[m];
+    P.el = P.nl * cm * cm;
+    P.er = P.nr * cm * cm;
+    max_e = std::max(max_e, std::max(P.el, P.er));
+    P.es = max_e;
+    P.xcap = std::max(max_e, (size_t)E->rdm_block_elems);
+    bool ok = cm * cm <= RDM_INT_MAX && ((P.nl + 31) / 32) * ((P.nr + 31) / 32) <= RDM_TILES_MAX;
+    auto pass_x = [&](int q0, int q1, int step) {
+      size_t nA = 1;
+      for (int q = q0; q != q1; q += step) {
+        const size_t cl = step > 0 ? s->chi[q] : s->chi[q + 1], cr = step > 0 ? s->chi[q + 1] : s->chi[q];
+        const int d = 2 * s->kap[q];
+        const size_t per = cl * nA * cr;
+        const size_t tiles = ((cl * nA + 31) / 32) * ((cr + 31) / 32);
+        const int w = rdm_block(d, per, P.xcap, tiles);
+        P.xs = std::max(P.xs, (size_t)w * per);
+        ok = ok && cl * nA <= RDM_INT_MAX && tiles * w <= RDM_TILES_MAX && 4 * nA * cr <= RDM_INT_MAX &&
+             ((cr + 31) / 32) * ((nA * cr + 31) / 32) * 2 <= RDM_TILES_MAX && (size_t)w * cl <= RDM_INT_MAX;
+        nA *= pos[q] >= 0 ? 4 : 1;
+      }
+    };
+    pass_x(0, m, 1);
+    pass_x(n - 1, m - 1, -1);
+    for (int q = 0; q < n; ++q) P.copies += sb[q] * (q < m ? (pos[q] >= 0 ? 4 : 1) : (pos[q] >= 0 ? 6 : 2));
+    P.total = 2 * align_up(P.es * 16) + align_up(P.xs * 16) + align_up(P.el * 16) + align_up(P.er * 16) +
+              2 * align_up(nout * 16) + P.copies + 4096;
+    return ok;
+  };
+  RdmPlan best;
+  for (int m = 0; m <= n; ++m) {
+    RdmPlan P;
+    if (plan(m, P) && (best.m < 0 || P.total < best.total)) best = P;
   }
-  HIP_TRY(launch_fill_identity1((double2*)env_a, st));
-  std::vector<std::pair<int, int>> label(1, {0, 0}), next;
+  if (best.m < 0) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix too large for these bonds");
+  const int m = best.m;
+  // large sweeps get a workspace of their own, released at the end; small ones use the arena
+  const bool own = best.total > ((size_t)1 << 30);
+  DevBuf work;
+  if (own) {
+    E->env.release();
+    if (hipMalloc(&work.p, best.total) != hipSuccess) {
+      work.p = nullptr;
+      (void)hipGetLastError();
+      return fail(HIPMPS_E_NOMEM, "reduced density matrix: workspace of " + std::to_string(best.total) + " bytes");
+    }
+    work.bytes = best.total;
+  } else {
+    RC_TRY(E->env.ensure(best.total));
+  }
+  struct Release {
+    DevBuf& b;
+    ~Release() { b.release(); }
+  } release_work{work};
+  char* cur = (char*)(own ? work.p : E->env.p);
+  auto take = [&](size_t bytes) {
+    char* p = cur;
+    cur += align_up(bytes);
+    return p;
+  };
+  char* p0 = take(best.es * 16);
+  char* p1 = take(best.es * 16);
+  char* xs = take(best.xs * 16);
+  char* el = take(best.el * 16);
+  char* er = take(best.er * 16);
+  char* cbuf = take(nout * 16);
+  char* obuf = take(nout * 16);
+  // the site copies below take exactly best.copies bytes
+  if ((size_t)(cur - (char*)(own ? work.p : E->env.p)) + best.copies > best.total)
+    return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix: workspace layout exceeds its plan");
+  std::vector<RdmSite> left, right;
   for (int q = 0; q < n; ++q) {
-    std::vector<EnvJob> jobs;
-    next.clear();
-    for (size_t j = 0; j < label.size(); ++j) {
-      const double2* ein = (const double2*)(env_a + j * mat);
-      if (pos[q] < 0) {
-        EnvJob job{s->sites[q].p, s->chi[q], s->chi[q + 1], ein, nullptr, (double2*)(env_b + next.size() * mat), 0};
-        job.kap = s->kap[q];
-        jobs.push_back(job);
-        next.push_back(label[j]);
+    const bool rev = q >= m, meas = pos[q] >= 0;
+    const int cl = s->chi[q], cr = s->chi[q + 1], d = 2 * s->kap[q];
+    const size_t bytes = site_elems(s, q) * s->es;
+    const void* ver[2] = {s->sites[q].p, nullptr};
+    if (meas)
+      for (int b = 0; b < 2; ++b) {
+        char* p = take(bytes);
+        HIP_TRY(hipMemcpyAsync(p, s->sites[q].p, bytes, hipMemcpyDeviceToDevice, st));
+        const double2 mb[4] = {make_double2(b == 0 ? 1.0 : 0.0, 0), make_double2(b == 1 ? 1.0 : 0.0, 0),
+                               make_double2(0, 0), make_double2(0, 0)};
+        HIP_TRY(launch_apply_1q(p, s->fp32, cl, s->kap[q] * cr, mb, st));
+        ver[b] = p;
+      }
+    RdmSite S{};
+    S.d = d;
+    S.meas = meas;
+    for (int v = 0; v < (meas ? 2 : 1); ++v) {
+      if (!rev) {
+        char* bra = take(bytes);  // [l][s][r] -> [s][l][r]
+        HIP_TRY(launch_swap_mid(ver[v], bra, 1, cl, d, cr, nullptr, s->fp32, st));
+        S.ket[v] = ver[v];
+        S.bra[v] = bra;
+        S.cl = cl;
+        S.cr = cr;
       } else {
-        for (int b = 0; b < 2; ++b)
-          for (int bp = 0; bp < 2; ++bp) {
-            EnvJob job{proj(q, b), s->chi[q], s->chi[q + 1], ein, nullptr, (double2*)(env_b + next.size() * mat), 0};
-            job.kap = s->kap[q];
-            job.Abra = proj(q, bp);
-            jobs.push_back(job);
-            next.push_back({label[j].first | (b << pos[q]), label[j].second | (bp << pos[q])});
-          }
+        char* bra = take(bytes);  // [l][s][r] -> [s][r][l]: the s-major copy with the bonds swapped
+        HIP_TRY(launch_swap_mid(ver[v], bra, 1, cl, d * cr, 1, nullptr, s->fp32, st));
+        char* ket = take(bytes);  // [s][r][l] -> [r][s][l]
+        HIP_TRY(launch_swap_mid(bra, ket, 1, d, cr, cl, nullptr, s->fp32, st));
+        S.ket[v] = ket;
+        S.bra[v] = bra;
+        S.cl = cr;
+        S.cr = cl;
       }
     }
-    for (size_t c0 = 0; c0 < jobs.size(); c0 += CH) {
-      std::vector<EnvJob> chunk(jobs.begin() + c0, jobs.begin() + std::min(jobs.size(), c0 + CH));
-      for (size_t t = 0; t < chunk.size(); ++t) chunk[t].X = (double2*)(xs + t * xmat);
-      RC_TRY(env_left(E, chunk, s->fp32));
-    }
-    std::swap(env_a, env_b);
-    label.swap(next);
+    (rev ? right : left).push_back(S);
   }
-  std::vector<double2> vals(label.size());
-  HIP_TRY(hipMemcpy2DAsync(vals.data(), sizeof(double2), env_a, mat, sizeof(double2), label.size(),
-                           hipMemcpyDeviceToHost, st));
+  std::reverse(right.begin(), right.end());
+  RC_TRY(rdm_half(E, left, s->fp32, s->es, p0, p1, xs, el, best.xcap));
+  RC_TRY(rdm_half(E, right, s->fp32, s->es, p0, p1, xs, er, best.xcap));
+  // join: both environments label-major, [A][l'][l], then one GEMM over (l', l)
+  const int cm = s->chi[m];
+  HIP_TRY(launch_swap_mid(el, p0, 1, cm, (int)best.nl, cm, nullptr, 0, st));
+  HIP_TRY(launch_swap_mid(er, p1, 1, cm, (int)best.nr, cm, nullptr, 0, st));
+  RC_TRY(E->hdesc.ensure(sizeof(GemmTask)));
+  RC_TRY(E->desc.ensure(sizeof(GemmTask)));
+  GemmTask* g = (GemmTask*)E->hdesc.p;
+  memset(g, 0, sizeof(GemmTask));
+  g->A = p0;
+  g->lda = cm * cm;
+  g->M = (int)best.nl;
+  g->K = cm * cm;
+  g->B = p1;
+  g->ldb = cm * cm;
+  g->N = (int)best.nr;
+  g->C = cbuf;
+  g->ldc = (int)best.nr;
+  g->tiles_m = (g->M + 31) / 32;
+  g->tiles_n = (g->N + 31) / 32;
+  HIP_TRY(hipMemcpyAsync(E->desc.p, g, sizeof(GemmTask), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_gemm((GemmTask*)E->desc.p, 1, g->tiles_m * g->tiles_n, 0, 3, 0, 0, 0, st));
+  // joined label A_L |A_R| + A_R: the right half's measured sites (in its sweep order, from
+  // site n-1) in the low base-4 digits, then the left half's
+  RdmPerm P{};
+  P.k = k;
+  int t = 0;
+  for (int q = n - 1; q >= m; --q)
+    if (pos[q] >= 0) P.pos[t++] = pos[q];
+  for (int q = 0; q < m; ++q)
+    if (pos[q] >= 0) P.pos[t++] = pos[q];
+  HIP_TRY(launch_rdm_scatter((const double2*)cbuf, (double2*)obuf, P, st));
+  HIP_TRY(hipMemcpyAsync(out, obuf, nout * 16, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  const size_t dim = (size_t)1 << k;
-  out.assign(dim * dim, make_double2(0, 0));
-  for (size_t j = 0; j < label.size(); ++j) out[label[j].first + dim * label[j].second] = vals[j];
-  if (total > ((size_t)1 << 30)) E->env.release();  // do not hold GiBs of environments afterwards
+  if (!own && best.total > ((size_t)256 << 20)) E->env.release();
   return 0;
 }
 
@@ -1665,13 +1877,14 @@ int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out) {
   out->gram32_sweeps = e->gram32_sweeps;
   out->lq_min_rank = e->lq_min_rank;
   out->big_lane_min_chi = e->big_min_chi;
+  out->rdm_block_elems = e->rdm_block_elems;
   return 0;
 }
 
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   if (!e || !t) return fail(HIPMPS_E_ARG, "null argument");
   if (t->jacobi_lanes < 1 || t->jacobi_lanes > hipmps_engine_s::MAX_LANES || t->lane_min_pairs < 0 ||
-      t->gram32_sweeps < 0 || t->lq_min_rank < 0 || t->big_lane_min_chi < 0)
+      t->gram32_sweeps < 0 || t->lq_min_rank < 0 || t->big_lane_min_chi < 0 || t->rdm_block_elems < 1)
     return fail(HIPMPS_E_ARG, "tuning values out of range");
   std::lock_guard<std::mutex> lock(e->mu);
   int made = 1;  // lanes whose stream exists
@@ -1681,6 +1894,7 @@ int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   e->gram32_sweeps = t->gram32_sweeps;
   e->lq_min_rank = t->lq_min_rank;
   e->big_min_chi = t->big_lane_min_chi;
+  e->rdm_block_elems = t->rdm_block_elems;
   return 0;
 }
 
@@ -1846,15 +2060,10 @@ int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out) {
 
 int hipmps_reduced_density_matrix(hipmps_state s, const int* qubits, int k, double* out, size_t cap) {
   if (!s || !qubits || !out || k < 1) return fail(HIPMPS_E_ARG, "bad arguments");
-  if (k <= 30 && cap < ((size_t)1 << (2 * k))) return fail(HIPMPS_E_ARG, "output buffer too small");
+  if (k > RDM_MAX_QUBITS) return fail(HIPMPS_E_UNSUPPORTED, "reduced density matrix of 1 to 15 qubits");
+  if (cap < ((size_t)1 << (2 * k))) return fail(HIPMPS_E_ARG, "output buffer too small");
   RC_TRY(state_flush(s));
-  std::vector<double2> v;
-  RC_TRY(measure_rdm(s, std::vector<int>(qubits, qubits + k), v));
-  for (size_t t = 0; t < v.size(); ++t) {
-    out[2 * t] = v[t].x;
-    out[2 * t + 1] = v[t].y;
-  }
-  return 0;
+  return measure_rdm(s, std::vector<int>(qubits, qubits + k), out);
 }
 
 int hipmps_expval_diag(hipmps_state s, const double* diag, double* out) {

@@ -166,6 +166,13 @@ hipError_t launch_trace_prod(const double2* const* d_a, const double2* const* d_
 hipError_t launch_check_finite(const void* p, size_t n_complex, int fp32, int* d_flag,
                                hipStream_t s);
 hipError_t launch_fill_identity1(double2* p, hipStream_t s);
+// reduced density matrix of k <= 15 qubits: E[A] (4^k labels, base-4 digit t = 2 b_t + b'_t)
+// scattered to out[x + 2^k x'], x = sum_t b_t 2^pos[t], x' = sum_t b'_t 2^pos[t]
+struct RdmPerm {
+  int k;
+  int pos[15];
+};
+hipError_t launch_rdm_scatter(const double2* E, double2* out, const RdmPerm& P, hipStream_t s);
 
 // ---- purified sites (Kraus leg), kraus.hip ------------------------------------------------
 // dst[l][b][a][m] = src[l][a][b][m] for m < M (M = *m_dyn when m_dyn is not null, at most

@@ -13,6 +13,7 @@
 #include "kernels.h"
 #include "device_util.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -1073,6 +1074,29 @@ hipError_t launch_check_finite(const void* p, size_t n_complex, int fp32, int* d
 __global__ void k_fill_identity1(double2* p) { p[0] = make_double2(1.0, 0.0); }
 hipError_t launch_fill_identity1(double2* p, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_identity1, dim3(1), dim3(1), 0, s, p);
+  return hipGetLastError();
+}
+
+// density-matrix readout: label A holds, in base-4 digit t, the ket (2s) and bra (1s) component
+// of the t-th measured site; element A goes to out[x + 2^k x'] with x, x' the components placed
+// at the bits P.pos[t]
+__global__ __launch_bounds__(256) void k_rdm_scatter(const double2* __restrict__ E, double2* __restrict__ out,
+                                                     size_t count, RdmPerm P) {
+  for (size_t a = (size_t)blockIdx.x * 256 + threadIdx.x; a < count; a += (size_t)gridDim.x * 256) {
+    size_t x = 0, xb = 0;
+    for (int t = 0; t < P.k; ++t) {
+      const size_t d = (a >> (2 * t)) & 3;
+      x |= (d >> 1) << P.pos[t];
+      xb |= (d & 1) << P.pos[t];
+    }
+    out[x + (xb << P.k)] = E[a];
+  }
+}
+
+hipError_t launch_rdm_scatter(const double2* E, double2* out, const RdmPerm& P, hipStream_t s) {
+  const size_t count = (size_t)1 << (2 * P.k);
+  const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_rdm_scatter, dim3(blocks), dim3(256), 0, s, E, out, count, P);
   return hipGetLastError();
 }
 

@@ -60,9 +60,10 @@ void HipPmpsVisitor::after_gate(xacc::Instruction& g) {
   }
 }
 
-// finalize (ExaTnPmpsVisitor.cpp:657-782): the outcome distribution of the measured qubits
-// (the diagonal of their reduced density matrix), then the shots drawn from it with the
-// model's readout errors
+// finalize (ExaTnPmpsVisitor.cpp:657-782): the density matrix the reference stores under
+// "density_matrix" (all qubits when n <= 15, :664-683; else the measured qubits, in qubit order,
+// :701-761), the outcome distribution of the measured qubits from its diagonal, then the shots
+// drawn from it with the model's readout errors
 void HipPmpsVisitor::finalize() {
   int rc = hipmps_flush(m_state);
   if (rc != 0 && rc != HIPMPS_E_CONVERGENCE) check(rc, "flush");
@@ -76,33 +77,38 @@ void HipPmpsVisitor::finalize() {
   const int k = (int)m_measured.size();
   if (k > MAX_QUBITS_FOR_MEASURE)
     xacc::error("Measure more than " + std::to_string(MAX_QUBITS_FOR_MEASURE) + " is not supported.");
-  std::vector<double> probs((size_t)1 << k), diag(2 * m_n);
-  double ez = 0.0;
-  for (size_t x = 0; x < probs.size(); ++x) {
-    std::fill(diag.begin(), diag.end(), 1.0);
-    int parity = 0;
-    for (int i = 0; i < k; ++i) {
-      const int b = (int)((x >> i) & 1);
-      parity ^= b;
-      const int q = m_measured[i];
-      // several Measure on one qubit: the outcome bits must agree
-      diag[2 * q + (1 - b)] = 0.0;
-    }
-    check(hipmps_expval_diag(m_state, diag.data(), &probs[x]), "outcome probability");
-    ez += parity ? -probs[x] : probs[x];
+  std::vector<int> dmq;
+  if (m_n <= MAX_QUBITS_FOR_MEASURE) {
+    for (int q = 0; q < m_n; ++q) dmq.push_back(q);
+  } else {
+    dmq = m_measured;
+    std::sort(dmq.begin(), dmq.end());
+    dmq.erase(std::unique(dmq.begin(), dmq.end()), dmq.end());
   }
+  const size_t dim = (size_t)1 << dmq.size();
+  // flattened ket index fastest, as the reference's; written straight into the result pairs
+  std::vector<std::pair<double, double>> flat(dim * dim);
+  static_assert(sizeof(std::pair<double, double>) == 2 * sizeof(double), "pair layout");
+  check(hipmps_reduced_density_matrix(m_state, dmq.data(), (int)dmq.size(), reinterpret_cast<double*>(flat.data()),
+                                      dim * dim),
+        "density matrix");
+  // outcome y of the measured qubits (bit i = qubit m_measured[i]); several Measure on one
+  // qubit read the same bit
+  std::vector<int> slot(k);
+  for (int i = 0; i < k; ++i)
+    slot[i] = (int)(std::lower_bound(dmq.begin(), dmq.end(), m_measured[i]) - dmq.begin());
+  std::vector<double> probs((size_t)1 << k, 0.0);
+  for (size_t x = 0; x < dim; ++x) {
+    size_t y = 0;
+    for (int i = 0; i < k; ++i) y |= ((x >> slot[i]) & 1) << i;
+    probs[y] += flat[x + dim * x].first;
+  }
+  buffer->addExtraInfo("density_matrix", flat);
+  flat.clear();
+  flat.shrink_to_fit();
   buffer->addExtraInfo("density-matrix-diagonal", probs);
-  std::vector<int> sorted(m_measured);
-  std::sort(sorted.begin(), sorted.end());
-  const bool distinct = std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end();
-  if (k <= 8 && distinct) {  // the reduced density matrix, flattened ket index fastest (:684-757)
-    const size_t dim = (size_t)1 << k;
-    std::vector<double> rho(2 * dim * dim);
-    check(hipmps_reduced_density_matrix(m_state, m_measured.data(), k, rho.data(), dim * dim), "density matrix");
-    std::vector<std::pair<double, double>> flat(dim * dim);
-    for (size_t t = 0; t < flat.size(); ++t) flat[t] = {rho[2 * t], rho[2 * t + 1]};
-    buffer->addExtraInfo("density_matrix", flat);
-  }
+  double ez = 0.0;
+  for (size_t y = 0; y < probs.size(); ++y) ez += (__builtin_popcountll(y) & 1) ? -probs[y] : probs[y];
   buffer->addExtraInfo("exp-val-z", ez);
   double tot = 0.0;
   for (double p : probs) tot += std::max(p, 0.0);

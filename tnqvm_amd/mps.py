@@ -399,7 +399,14 @@ def run_noisy(engine: Engine, n: int, gates, noise=None, measured=(), shots: int
                 bits.append("1" if b else "0")
             key = "".join(bits)
             counts[key] = counts.get(key, 0) + 1
-        return {"probabilities": probs, "counts": counts, "expval_z": s.expval_z_all(),
-                "bonds": s.bonds(), "kraus_dims": s.kraus_dims(), "trace": s.norm()}
+        out = {"probabilities": probs, "counts": counts, "expval_z": s.expval_z_all(),
+               "bonds": s.bonds(), "kraus_dims": s.kraus_dims(), "trace": s.norm()}
+        if measured:
+            # what the reference stores as "density_matrix": all qubits when n <= 15 (:664-683),
+            # else the distinct measured qubits in qubit order (:701-761); rho[x, x']
+            dmq = list(range(n)) if n <= 15 else sorted(set(measured))
+            out["density_matrix"] = s.reduced_density_matrix(dmq)
+            out["density_matrix_qubits"] = dmq
+        return out
     finally:
         s.close()
