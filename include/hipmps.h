@@ -161,6 +161,8 @@ int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* i
  * complex Kraus operators (row-major, 8 doubles each, 1 <= n_ops <= HIPMPS_MAX_KRAUS_OPS).
  * The Kraus leg is compressed as the reference does: vectors with partial norm below 1e-9
  * dropped, at most min(2 chi_l chi_r, 2 kap) kept.  Gates and channels apply in call order. */
+/* The channel is queued like a gate: channels on distinct sites run together after the open
+ * wave of gates, and an error of the compression is returned by the call that runs it. */
 #define HIPMPS_MAX_KRAUS_OPS 16
 int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats);
 int hipmps_kraus_dims(hipmps_state s, int* out_n); /* kap_k per site */

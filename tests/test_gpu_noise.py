@@ -288,3 +288,30 @@ def test_density_matrix_wide_kraus(eng, block):
     finally:
         eng.set_tuning(**base)
     s.close()
+
+
+def test_channels_queued_equal_one_at_a_time(eng):
+    """Channels queue after the open wave of gates and run together; flushing after every
+    channel instead (kraus_dims) runs them one at a time.  Both orders apply the same
+    operations to the same tensors (queued channels only pass gates on other sites), so the
+    results agree to rounding, pending 1-qubit gates folded into the operators included."""
+    say("channels queued vs one at a time")
+    n = 6
+    ops = []
+    for g in C.u3_cnot_line(n, 4, seed=91):
+        ops.append(("g", g))
+        for q in g["qubits"]:
+            ops.append(("c", q, [N.amplitude_damping(0.07), N.depolarizing(0.1), N.phase_damping(0.2)][q % 3]))
+    a = run_device(eng, n, ops, "fp64")
+    b = mps.MpsState(eng, n, cutoff=1e-9)
+    for op in ops:
+        if op[0] == "g":
+            b.run([op[1]])
+        else:
+            b.apply_kraus(op[1], op[2])
+            b.kraus_dims()
+    assert list(a.kraus_dims()) == list(b.kraus_dims())
+    assert np.abs(a.expval_z_all() - b.expval_z_all()).max() < 1e-12
+    assert abs(a.norm() - b.norm()) < 1e-12
+    a.close()
+    b.close()

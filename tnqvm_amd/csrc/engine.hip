@@ -226,6 +226,14 @@ struct hipmps_state_s {
   };
   std::vector<Q2> wave;
   std::vector<char> busy;
+  // noise channels queued after the open wave of gates, on distinct sites; they run as one
+  // wave of compressions after the gates
+  struct QK {
+    int q, n_ops;
+    cd K[4 * HIPMPS_MAX_KRAUS_OPS];
+  };
+  std::vector<QK> cwave;
+  std::vector<char> cbusy;
   double discarded = 0.0;
   double fid = 1.0;
   int maxb = 1;
@@ -352,6 +360,7 @@ int state_init_product(hipmps_state_s* s) {
   s->pend.assign(s->n, {cd(1), cd(0), cd(0), cd(1)});
   s->has_pend.assign(s->n, 0);
   s->busy.assign(s->n, 0);
+  s->cbusy.assign(s->n, 0);
   s->wave.clear();
   s->discarded = 0.0;
   s->fid = 1.0;
@@ -854,21 +863,26 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
 // ------------------------------------------------------------------------------------
 // per-state gate queue
 // ------------------------------------------------------------------------------------
+int run_kraus_wave(hipmps_state_s* s);
+
+// the open wave of gates, then the channels queued after it
 int state_run_wave(hipmps_state_s* s) {
-  if (s->wave.empty()) return 0;
-  std::vector<WaveItem> items;
-  items.reserve(s->wave.size());
-  for (auto& q : s->wave) {
-    WaveItem it{};
-    it.s = s;
-    it.i = q.i;
-    it.G = q.G;
-    items.push_back(it);
+  if (!s->wave.empty()) {
+    std::vector<WaveItem> items;
+    items.reserve(s->wave.size());
+    for (auto& q : s->wave) {
+      WaveItem it{};
+      it.s = s;
+      it.i = q.i;
+      it.G = q.G;
+      items.push_back(it);
+    }
+    int rc = run_wave(s->e, items);
+    s->wave.clear();
+    std::fill(s->busy.begin(), s->busy.end(), 0);
+    if (rc != 0) return rc;
   }
-  int rc = run_wave(s->e, items);
-  s->wave.clear();
-  std::fill(s->busy.begin(), s->busy.end(), 0);
-  return rc;
+  return run_kraus_wave(s);
 }
 
 // Queue a 2-qubit gate; returns 1 if the gate conflicts with the open wave (caller runs
@@ -876,7 +890,8 @@ int state_run_wave(hipmps_state_s* s) {
 int queue_2q(hipmps_state_s* s, int q0, int q1, const cd* M, bool* conflict) {
   const int i = std::min(q0, q1);
   *conflict = false;
-  if (s->busy[i] || s->busy[i + 1]) {
+  // a gate on a site with a queued channel comes after that channel: next wave
+  if (s->busy[i] || s->busy[i + 1] || s->cbusy[i] || s->cbusy[i + 1]) {
     *conflict = true;
     return 0;
   }
@@ -944,104 +959,161 @@ int state_flush(hipmps_state_s* s) {
 // leaves rho unchanged).  Kept: the rows up to the first whose partial norm is below 1e-9 (that
 // row excluded, ExaTnPmpsVisitor.cpp:1060-1072), at most min(2 chiL chiR, 2 kap) of them, the
 // reference's new Kraus extent (:985-996).
+// Channels are queued (one per site) after the open wave of gates and compressed together:
+// one expansion per channel, then one Jacobi launch per round for all of them, one selection
+// and one readback of the new extents.  A 1-qubit gate pending on the site when its channel
+// is queued is folded into the operators (K_o P); a gate on a site with a queued channel opens
+// the next wave.
 constexpr double KRAUS_EPS = 1e-9;
 
-int apply_kraus(hipmps_state_s* s, int q, int n_ops, const cd* K) {
+int run_kraus_wave(hipmps_state_s* s) {
+  if (s->cwave.empty()) return 0;
   hipmps_engine E = s->e;
   const hipStream_t st = E->stream;
-  RC_TRY(state_flush(s));  // every gate before the channel, pending 1-qubit gates on q included
-  const int cl = s->chi[q], cr = s->chi[q + 1], kap = s->kap[q];
-  const int rows = kap * n_ops;
-  if (rows > 4096) return fail(HIPMPS_E_UNSUPPORTED, "Kraus leg above 4096 rows");
-  const int r_pad = (rows + JP - 1) / JP * JP;
-  const size_t c = (size_t)2 * cl * cr;
-  size_t off = 0;
-  const size_t off_x = off;
-  off = align_up(off + (size_t)r_pad * c * 16);
-  const size_t off_v = off;
-  off = align_up(off + (size_t)r_pad * (8 * 5 + 4));
-  const size_t off_k = off;
-  off = align_up(off + (size_t)n_ops * 4 * 16);
+  const int C = (int)s->cwave.size();
+  struct Ch {
+    int q, cl, cr, kap, n_ops, rows, r_pad;
+    size_t c, off_x, off_v, off_k;
+  };
+  std::vector<Ch> ch(C);
+  size_t off = 0, kbytes = 0;
+  int max_rows = 1;
+  for (int i = 0; i < C; ++i) {
+    Ch& h = ch[i];
+    h.q = s->cwave[i].q;
+    h.n_ops = s->cwave[i].n_ops;
+    h.cl = s->chi[h.q];
+    h.cr = s->chi[h.q + 1];
+    h.kap = s->kap[h.q];
+    h.rows = h.kap * h.n_ops;
+    if (h.rows > 4096) return fail(HIPMPS_E_UNSUPPORTED, "Kraus leg above 4096 rows");
+    h.r_pad = (h.rows + JP - 1) / JP * JP;
+    h.c = (size_t)2 * h.cl * h.cr;
+    h.off_x = off;
+    off = align_up(off + (size_t)h.r_pad * h.c * 16);
+    h.off_v = off;
+    off = align_up(off + (size_t)h.r_pad * (8 * 5 + 4));
+    h.off_k = off;
+    off = align_up(off + (size_t)h.n_ops * 4 * 16);
+    kbytes += (size_t)h.n_ops * 4 * sizeof(double2);
+    max_rows = std::max(max_rows, h.rows);
+  }
   RC_TRY(E->ws.ensure(off));
   char* ws = (char*)E->ws.p;
-  double2* X = (double2*)(ws + off_x);
-  RC_TRY(E->small.ensure(align_up(3 * sizeof(int)) + WS_N * sizeof(double)));
-  int* d_counts = (int*)E->small.p;  // rotated, visited
-  int* d_K = d_counts + 2;
-  double* d_w = (double*)((char*)E->small.p + align_up(3 * sizeof(int)));
-  const size_t desc_bytes = align_up(sizeof(SplitTask)) + align_up(sizeof(JacTask));
+  const size_t cnt_bytes = align_up((size_t)3 * C * sizeof(int));
+  RC_TRY(E->small.ensure(cnt_bytes + (size_t)C * WS_N * sizeof(double)));
+  int* d_counts = (int*)E->small.p;  // rotated, visited per channel
+  int* d_K = d_counts + 2 * C;       // kept Kraus vectors per channel
+  double* d_w = (double*)((char*)E->small.p + cnt_bytes);
+  const size_t sp_bytes = align_up((size_t)C * sizeof(SplitTask)), jt_bytes = align_up((size_t)C * sizeof(JacTask));
+  const size_t desc_bytes = sp_bytes + jt_bytes;
   RC_TRY(E->desc.ensure(desc_bytes));
-  const size_t kbytes = (size_t)n_ops * 4 * sizeof(double2);
-  RC_TRY(E->hdesc.ensure(desc_bytes + kbytes + align_up(3 * sizeof(int)) + WS_N * sizeof(double)));
+  const size_t h_jac = desc_bytes, h_k = h_jac + jt_bytes, h_cnt = h_k + align_up(kbytes),
+               h_w = h_cnt + cnt_bytes;
+  RC_TRY(E->hdesc.ensure(h_w + (size_t)C * WS_N * sizeof(double)));
   char* hd = (char*)E->hdesc.p;
   SplitTask* hs = (SplitTask*)hd;
-  JacTask* hj = (JacTask*)(hd + align_up(sizeof(SplitTask)));
-  double2* hk = (double2*)(hd + desc_bytes);
-  int* hcount = (int*)(hd + desc_bytes + kbytes);
-  double* hw = (double*)(hd + desc_bytes + kbytes + align_up(3 * sizeof(int)));
-  for (int t = 0; t < 4 * n_ops; ++t) hk[t] = make_double2(K[t].real(), K[t].imag());
-  memset(hs, 0, sizeof(SplitTask));
-  char* v = ws + off_v;
-  hs->Y = X;
-  hs->r = rows;
-  hs->c = (int)c;
-  hs->ldy = (int)c;
-  hs->m = rows;
-  hs->n = (int)c;
-  hs->rownorm = (double*)v;
-  hs->sig = (double*)(v + 8 * r_pad);
-  hs->scale_y = (double*)(v + 16 * r_pad);
-  hs->scale_w = (double*)(v + 24 * r_pad);
-  hs->inv_sig = (double*)(v + 32 * r_pad);
-  hs->perm = (int*)(v + 40 * r_pad);
-  hs->K = d_K;
-  hs->wstats = d_w;
-  hs->cutoff = KRAUS_EPS;
-  hs->max_bond = (int)std::min<size_t>(c, (size_t)2 * kap);
-  hs->pn_l2 = s->t.partial_norm == HIPMPS_PN_L2;
-  hs->site_fp32 = s->fp32;
-  hs->keep_rule = 1;
-  memset(hj, 0, sizeof(JacTask));
-  hj->Y = X;
-  hj->ldy = (int)c;
-  hj->r = rows;
-  hj->r_pad = r_pad;
-  hj->c = (int)c;
-  hj->c_tot = (int)c;
-  hj->nb = r_pad / JB;
-  hj->pair_begin = 0;
-  hj->counter = d_counts;
-  hj->visited = d_counts + 1;
-  hj->tol = s->fp32 ? E->tol_mixed : E->tol;
+  JacTask* hj = (JacTask*)(hd + sp_bytes);
+  JacTask* hja = (JacTask*)(hd + h_jac);  // active list of a sweep
+  double2* hk = (double2*)(hd + h_k);
+  int* hcount = (int*)(hd + h_cnt);
+  double* hw = (double*)(hd + h_w);
+  memset(hs, 0, desc_bytes);
+  size_t ko = 0;
+  for (int i = 0; i < C; ++i) {
+    const Ch& h = ch[i];
+    for (int t = 0; t < 4 * h.n_ops; ++t)
+      hk[ko + t] = make_double2(s->cwave[i].K[t].real(), s->cwave[i].K[t].imag());
+    char* v = ws + h.off_v;
+    double2* X = (double2*)(ws + h.off_x);
+    SplitTask& S = hs[i];
+    S.Y = X;
+    S.r = h.rows;
+    S.c = (int)h.c;
+    S.ldy = (int)h.c;
+    S.m = h.rows;
+    S.n = (int)h.c;
+    S.rownorm = (double*)v;
+    S.sig = (double*)(v + 8 * h.r_pad);
+    S.scale_y = (double*)(v + 16 * h.r_pad);
+    S.scale_w = (double*)(v + 24 * h.r_pad);
+    S.inv_sig = (double*)(v + 32 * h.r_pad);
+    S.perm = (int*)(v + 40 * h.r_pad);
+    S.K = d_K + i;
+    S.wstats = d_w + (size_t)i * WS_N;
+    S.cutoff = KRAUS_EPS;
+    S.max_bond = (int)std::min<size_t>(h.c, (size_t)2 * h.kap);
+    S.pn_l2 = s->t.partial_norm == HIPMPS_PN_L2;
+    S.site_fp32 = s->fp32;
+    S.keep_rule = 1;
+    JacTask& J = hj[i];
+    J.Y = X;
+    J.ldy = (int)h.c;
+    J.r = h.rows;
+    J.r_pad = h.r_pad;
+    J.c = (int)h.c;
+    J.c_tot = (int)h.c;
+    J.nb = h.r_pad / JB;
+    J.counter = d_counts + 2 * i;
+    J.visited = d_counts + 2 * i + 1;
+    J.tol = s->fp32 ? E->tol_mixed : E->tol;
+    ko += 4 * (size_t)h.n_ops;
+  }
   char* dd = (char*)E->desc.p;
   const SplitTask* dsp = (const SplitTask*)dd;
-  const JacTask* djt = (const JacTask*)(dd + align_up(sizeof(SplitTask)));
-  HIP_TRY(hipMemcpyAsync(dd, hd, desc_bytes, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ws + off_k, hk, kbytes, hipMemcpyHostToDevice, st));
-  HIP_TRY(launch_kraus_expand(s->sites[q].p, s->fp32, cl, kap, cr, (const double2*)(ws + off_k), n_ops, X, r_pad, st));
-  bool converged = false;
-  for (int sweep = 0; sweep < E->max_sweeps && !converged; ++sweep) {
-    HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * sizeof(int), st));
-    for (int round = 0; round < hj->nb - 1; ++round)
-      HIP_TRY(launch_jacobi_round(djt, 1, hj->nb / 2, round, E->inner_sweeps, st));
-    HIP_TRY(hipMemcpyAsync(hcount, d_counts, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    converged = hcount[0] == 0;
+  JacTask* djt = (JacTask*)(dd + sp_bytes);
+  HIP_TRY(hipMemcpyAsync(dd, hd, sp_bytes, hipMemcpyHostToDevice, st));
+  ko = 0;
+  for (int i = 0; i < C; ++i) {
+    const Ch& h = ch[i];
+    HIP_TRY(hipMemcpyAsync(ws + h.off_k, hk + ko, (size_t)h.n_ops * 4 * sizeof(double2), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_kraus_expand(s->sites[h.q].p, s->fp32, h.cl, h.kap, h.cr, (const double2*)(ws + h.off_k),
+                                h.n_ops, (double2*)(ws + h.off_x), h.r_pad, st));
+    ko += 4 * (size_t)h.n_ops;
   }
-  if (!converged) s->status = HIPMPS_E_CONVERGENCE;
-  HIP_TRY(launch_rownorm(dsp, 1, rows, st));
-  HIP_TRY(launch_select(dsp, 1, st));
-  HIP_TRY(hipMemcpyAsync(hcount + 2, d_K, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(hw, d_w, WS_N * sizeof(double), hipMemcpyDeviceToHost, st));
+  // sweeps over the channels not yet converged, one launch per round for all of them
+  std::vector<int> active(C);
+  for (int i = 0; i < C; ++i) active[i] = i;
+  for (int sweep = 0; sweep < E->max_sweeps && !active.empty(); ++sweep) {
+    int pairs = 0, rounds = 0;
+    for (size_t a = 0; a < active.size(); ++a) {
+      hja[a] = hj[active[a]];
+      hja[a].pair_begin = pairs;
+      pairs += hja[a].nb / 2;
+      rounds = std::max(rounds, hja[a].nb - 1);
+    }
+    HIP_TRY(hipMemcpyAsync(djt, hja, active.size() * sizeof(JacTask), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(d_counts, 0, 2 * (size_t)C * sizeof(int), st));
+    for (int round = 0; round < rounds; ++round)
+      HIP_TRY(launch_jacobi_round(djt, (int)active.size(), pairs, round, E->inner_sweeps, st));
+    HIP_TRY(hipMemcpyAsync(hcount, d_counts, 2 * (size_t)C * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int> next;
+    for (int i : active)
+      if (hcount[2 * i] != 0) next.push_back(i);
+    active.swap(next);
+  }
+  if (!active.empty()) s->status = HIPMPS_E_CONVERGENCE;
+  HIP_TRY(launch_rownorm(dsp, C, max_rows, st));
+  HIP_TRY(launch_select(dsp, C, st));
+  HIP_TRY(hipMemcpyAsync(hcount + 2 * C, d_K, (size_t)C * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hw, d_w, (size_t)C * WS_N * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  const int Kn = hcount[2];
-  if (Kn < 1 || Kn > rows) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
-  if (hw[5] != 0.0 || !std::isfinite(hw[0] + hw[1])) s->status = HIPMPS_E_NUMERIC;
-  RC_TRY(ensure_site(s, q, (size_t)cl * 2 * Kn * cr, false, nullptr));
-  HIP_TRY(launch_kraus_gather(X, (int)c, (const int*)(v + 40 * r_pad), d_K, Kn, s->sites[q].p, s->fp32, cl, cr, st));
+  for (int i = 0; i < C; ++i) {
+    const Ch& h = ch[i];
+    const int Kn = hcount[2 * C + i];
+    if (Kn < 1 || Kn > h.rows) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
+    const double* w = hw + (size_t)i * WS_N;
+    if (w[5] != 0.0 || !std::isfinite(w[0] + w[1])) s->status = HIPMPS_E_NUMERIC;
+    RC_TRY(ensure_site(s, h.q, (size_t)h.cl * 2 * Kn * h.cr, false, nullptr));
+    HIP_TRY(launch_kraus_gather((double2*)(ws + h.off_x), (int)h.c, (const int*)(ws + h.off_v + 40 * (size_t)h.r_pad),
+                                d_K + i, Kn, s->sites[h.q].p, s->fp32, h.cl, h.cr, st));
+    s->kap[h.q] = Kn;
+    s->kraus_ops++;
+  }
   HIP_TRY(hipStreamSynchronize(st));
-  s->kap[q] = Kn;
-  s->kraus_ops++;
+  s->cwave.clear();
+  std::fill(s->cbusy.begin(), s->cbusy.end(), 0);
   return 0;
 }
 
@@ -1991,10 +2063,24 @@ int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats) {
   if (!s || !mats || q < 0 || q >= s->n || n_ops < 1 || n_ops > HIPMPS_MAX_KRAUS_OPS)
     return fail(HIPMPS_E_ARG, "bad arguments");
   if (!all_finite(mats, (size_t)8 * n_ops)) return fail(HIPMPS_E_NUMERIC, "Kraus operator holds NaN or Inf");
-  std::vector<cd> K(4 * (size_t)n_ops);
-  for (size_t t = 0; t < K.size(); ++t) K[t] = cd(mats[2 * t], mats[2 * t + 1]);
   RC_TRY(set_device(s->e));
-  return apply_kraus(s, q, n_ops, K.data());
+  if (s->kap[q] * n_ops > 4096) {
+    RC_TRY(state_flush(s));  // the Kraus extent of q after the channels already queued
+    if (s->kap[q] * n_ops > 4096) return fail(HIPMPS_E_UNSUPPORTED, "Kraus leg above 4096 rows");
+  }
+  if (s->cbusy[q]) RC_TRY(state_run_wave(s));  // one queued channel per site
+  hipmps_state_s::QK ch;
+  ch.q = q;
+  ch.n_ops = n_ops;
+  for (int t = 0; t < 4 * n_ops; ++t) ch.K[t] = cd(mats[2 * t], mats[2 * t + 1]);
+  if (s->has_pend[q]) {  // 1-qubit gates still pending on q act before the channel: K_o P
+    for (int o = 0; o < n_ops; ++o) mat_mul2(ch.K + 4 * o, s->pend[q].data(), ch.K + 4 * o);
+    s->has_pend[q] = 0;
+    s->pend[q] = {cd(1), cd(0), cd(0), cd(1)};
+  }
+  s->cwave.push_back(ch);
+  s->cbusy[q] = 1;
+  return 0;
 }
 
 int hipmps_kraus_dims(hipmps_state s, int* out) {

@@ -363,7 +363,7 @@ class MpsState:
 
 
 def run_noisy(engine: Engine, n: int, gates, noise=None, measured=(), shots: int = 1024, seed=None,
-              precision: str = "fp64"):
+              precision: str = "fp64", max_bond: int = 0):
     """A noisy circuit the way the reference's purified-MPS visitor runs one
     (ExaTnPmpsVisitor.cpp): each gate followed by the single-qubit channels the noise model
     gives for it (:775-829), bonds cut at 1e-9 with no cap (truncateSvdTensors' default,
@@ -372,8 +372,9 @@ def run_noisy(engine: Engine, n: int, gates, noise=None, measured=(), shots: int
     for a noisy run, :490).  String character i is the outcome of measured[i].
 
     noise: None or an object with channels(gate) -> [(qubit, Kraus ops)] and
-    readout_error(qubit) -> (P(0 | 1), P(1 | 0)) (tnqvm_amd.noise.NoiseModel)."""
-    s = MpsState(engine, n, cutoff=1e-9, precision=precision)
+    readout_error(qubit) -> (P(0 | 1), P(1 | 0)) (tnqvm_amd.noise.NoiseModel).
+    max_bond: 0 (the reference: no cap) or a cap on the bonds, as the visitor's max-bond-dim."""
+    s = MpsState(engine, n, cutoff=1e-9, max_bond=max_bond, precision=precision)
     try:
         for g in gates:
             if g["name"] in ("Measure", "I"):
