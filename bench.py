@@ -21,7 +21,13 @@ Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
                      plain path one at a time: circuits/s and p50/p99 latency for both
   latency_ms_p50/p99 the batched serving latencies (arrival to result)
   roofline           k_jacobi_round (the dominant kernel), FP64 MFMA bound
-  cpu_baseline       the FP64 CPU oracle on the host cores at chi = 256
+  noisy              the purified-MPS (noisy) path: a 40-qubit u3/CNOT circuit of depth 4 with the
+                     ibmqx2 backend's single-qubit channels after every gate (qubit q takes the
+                     channels of backend qubit q mod 5), no bond cap, cut 1e-9: wall time per
+                     circuit with its <Z> read back
+  cpu_baseline       the FP64 CPU oracle on the host cores at chi = 256; under "noisy", the
+                     restated purified-MPS algorithm on the same noisy circuit, with the largest
+                     <Z> difference from the GPU result
 """
 from __future__ import annotations
 
@@ -229,6 +235,62 @@ def cpu_baseline(seconds=12.0):
                       f"18-site FP64 MPS, {el:.1f} s, oracle/mps_oracle.cpp with {threads} OpenMP threads"}
 
 
+NOISY_N, NOISY_DEPTH = 40, 4
+
+
+def noisy_workload():
+    """The noisy circuit and its channels: [(gate, [(qubit, Kraus ops)])]."""
+    from tnqvm_amd import noise as N
+    with open(os.path.join(ROOT, "tests", "golden", "noise", "ibmqx2.json")) as f:
+        model = N.NoiseModel.from_backend_json(f.read())
+    ops = []
+    for g in C.u3_cnot_line(NOISY_N, NOISY_DEPTH, seed=5):
+        ops.append((g, [(q, model.one_qubit[q % 5]) for q in g["qubits"] if model.one_qubit.get(q % 5)]))
+    return ops
+
+
+def noisy_gpu(eng, ops, reps=5):
+    """Median wall time of the noisy circuit on the device (channels queued after each wave of
+    gates), <Z> of every qubit read back inside the timed region."""
+    times, ez, kd, bonds = [], None, None, None
+    for _ in range(reps + 1):  # the first run warms the arena
+        s = mps.MpsState(eng, NOISY_N, cutoff=1e-9)
+        t0 = time.perf_counter()
+        for g, chans in ops:
+            s.run([g])
+            for q, k in chans:
+                s.apply_kraus(q, k)
+        ez = s.expval_z_all()
+        times.append(time.perf_counter() - t0)
+        kd, bonds = int(max(s.kraus_dims())), int(max(s.bonds()))
+        s.close()
+    return float(np.median(times[1:])), ez, kd, bonds
+
+
+def cpu_baseline_noisy(ops, ez_gpu, seconds=3.0):
+    """The restated purified-MPS algorithm (oracle/noise.py, numpy) on the same noisy circuit."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle.noise import PurifiedMps
+    runs = 0
+    with threadpool_limits(limits=1):  # one core: the BLAS under numpy's SVDs single-threaded
+        t0 = time.perf_counter()
+        while True:
+            pm = PurifiedMps(NOISY_N)
+            for g, chans in ops:
+                pm.gate(mps.matrix_of(g), g["qubits"])
+                for q, k in chans:
+                    pm.channel(q, k)
+            runs += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or runs >= 20:
+                break
+        ez = pm.expval_z()
+    return {"value": runs / el, "unit": "noisy circuits/s", "cores": 1, "kind": "port",
+            "sample": f"{runs} runs of the noisy circuit, {el:.1f} s, oracle/noise.py PurifiedMps (numpy, one BLAS thread)",
+            "max_abs_dz_vs_gpu": float(np.abs(ez - ez_gpu).max())}
+
+
 def calibrate_copy(nbytes: int, reps: int = 4):
     """Plain device-to-device copies of a known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings (run under the profiler: bench.py --calibrate-copy BYTES)."""
@@ -257,6 +319,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-noisy", action="store_true")
     ap.add_argument("--uneven-circuits", type=int, default=256, help="circuits of the uneven batch")
     ap.add_argument("--serve-circuits", type=int, default=96, help="brickwork circuits in the serving stream")
     ap.add_argument("--serve-grid", type=int, default=1, help="6x6 grid circuits in the serving stream")
@@ -414,10 +477,25 @@ def main():
                                                "circuit (one circuit at a time)"}
             serving["speedup_vs_plain"] = round(pspan / span, 3)
         out["serving"] = serving
+    noisy_ops, noisy_ez = None, None
+    if not args.no_noisy:
+        noisy_ops = noisy_workload()
+        nt, noisy_ez, kd, bonds = noisy_gpu(eng, noisy_ops)
+        nch = sum(len(c) for _, c in noisy_ops)
+        log(f"noisy: {len(noisy_ops)} gates, {nch} channels, {nt * 1e3:.1f} ms per circuit (Kraus legs up to {kd}, "
+            f"bonds up to {bonds})")
+        out["noisy"] = {"workload": f"{NOISY_N}-qubit u3/CNOT line, depth {NOISY_DEPTH}, ibmqx2 single-qubit "
+                                    "channels after every gate (backend qubit q mod 5), cut 1e-9, no bond cap",
+                        "gates": len(noisy_ops), "channels": nch, "ms_per_circuit": round(nt * 1e3, 2),
+                        "circuits_per_s": round(1.0 / nt, 3), "max_kraus": kd, "max_bond": bonds}
     eng.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")
         out["cpu_baseline"] = cpu_baseline()
+        if noisy_ops is not None:
+            log("cpu baseline, noisy (purified-MPS oracle)")
+            out["cpu_baseline"]["noisy"] = cpu_baseline_noisy(noisy_ops, noisy_ez)
+            out["noisy"]["speedup_vs_cpu"] = round(out["noisy"]["circuits_per_s"] / out["cpu_baseline"]["noisy"]["value"], 2)
     print(json.dumps(out), flush=True)
 
 

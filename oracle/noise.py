@@ -194,6 +194,20 @@ class PurifiedMps:
         self.Q[q] = np.ascontiguousarray(Qn)
         return self
 
+    def expval_z(self):
+        """Tr(rho Z_q) for every qubit through left and right environments (any n), the Kraus
+        leg traced with the physical one."""
+        left = [np.ones((1, 1), dtype=np.complex128)]
+        for Q in self.Q:
+            left.append(np.einsum("lm,lpkr,mpks->rs", left[-1], Q, Q.conj()))
+        right = [np.ones((1, 1), dtype=np.complex128)]
+        for Q in reversed(self.Q):
+            right.append(np.einsum("rs,lpkr,mpks->lm", right[-1], Q, Q.conj()))
+        right = right[::-1]  # right[q]: the sites q.. n-1 contracted, at bond q
+        z = np.array([1.0, -1.0])
+        return np.array([np.einsum("lm,lpkr,p,mpks,rs->", left[q], Q, z, Q.conj(), right[q + 1]).real
+                         for q, Q in enumerate(self.Q)])
+
     def density_matrix(self):
         """rho = sum over Kraus legs of |psi><psi| (small n only)."""
         R = np.ones((1, 1, 1, 1), dtype=np.complex128)

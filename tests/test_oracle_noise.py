@@ -95,6 +95,20 @@ def test_purified_mps_matches_density_matrix(pn, n, depth):
     assert np.abs(pm.density_matrix() - dm.rho).max() < 1e-8
 
 
+def test_purified_mps_expval_z_matches_its_density_matrix():
+    """The environment route to <Z_q> (any n) against the dense route on the same state,
+    with Kraus legs of several extents and a sign pattern no product state has."""
+    n = 5
+    ops = _noisy(n, 4, 37, [N.amplitude_damping(0.2), N.depolarizing(0.1), N.random_channel(np.random.default_rng(3), 3)])
+    pm = _run(PurifiedMps(n), ops)
+    assert max(pm.kraus_dims()) > 1
+    dm = pm.as_density()
+    ez = pm.expval_z()
+    for q in range(n):
+        assert abs(ez[q] - dm.z(q)) < 1e-12
+    assert np.ptp([dm.z(q) for q in range(n)]) > 0.05
+
+
 def test_purified_mps_uncapped_matches_depolarizing():
     n = 4
     ops = _noisy(n, 4, 29, [N.depolarizing(0.08), N.random_channel(np.random.default_rng(2), 4)])
