@@ -971,8 +971,11 @@ int run_kraus_wave(hipmps_state_s* s) {
   hipmps_engine E = s->e;
   const hipStream_t st = E->stream;
   const int C = (int)s->cwave.size();
+  // adj: more rows (kap n_ops) than columns (2 cl cr); the Jacobi then runs on the rows of
+  // [X^H | I] (jr = c rows, ld = rows + c), a sweep of few rounds instead of many
   struct Ch {
-    int q, cl, cr, kap, n_ops, rows, r_pad;
+    int q, cl, cr, kap, n_ops, rows, r_pad, jr, jc, ld;
+    bool adj;
     size_t c, off_x, off_v, off_k;
   };
   std::vector<Ch> ch(C);
@@ -987,16 +990,20 @@ int run_kraus_wave(hipmps_state_s* s) {
     h.kap = s->kap[h.q];
     h.rows = h.kap * h.n_ops;
     if (h.rows > 4096) return fail(HIPMPS_E_UNSUPPORTED, "Kraus leg above 4096 rows");
-    h.r_pad = (h.rows + JP - 1) / JP * JP;
     h.c = (size_t)2 * h.cl * h.cr;
+    h.adj = (size_t)h.rows > h.c;
+    h.jr = h.adj ? (int)h.c : h.rows;
+    h.jc = h.adj ? h.rows : (int)h.c;
+    h.ld = h.adj ? h.rows + (int)h.c : (int)h.c;
+    h.r_pad = (h.jr + JP - 1) / JP * JP;
     h.off_x = off;
-    off = align_up(off + (size_t)h.r_pad * h.c * 16);
+    off = align_up(off + (size_t)h.r_pad * h.ld * 16);
     h.off_v = off;
     off = align_up(off + (size_t)h.r_pad * (8 * 5 + 4));
     h.off_k = off;
     off = align_up(off + (size_t)h.n_ops * 4 * 16);
     kbytes += (size_t)h.n_ops * 4 * sizeof(double2);
-    max_rows = std::max(max_rows, h.rows);
+    max_rows = std::max(max_rows, h.jr);
   }
   RC_TRY(E->ws.ensure(off));
   char* ws = (char*)E->ws.p;
@@ -1028,11 +1035,11 @@ int run_kraus_wave(hipmps_state_s* s) {
     double2* X = (double2*)(ws + h.off_x);
     SplitTask& S = hs[i];
     S.Y = X;
-    S.r = h.rows;
-    S.c = (int)h.c;
-    S.ldy = (int)h.c;
-    S.m = h.rows;
-    S.n = (int)h.c;
+    S.r = h.jr;
+    S.c = h.jc;
+    S.ldy = h.ld;
+    S.m = h.jr;
+    S.n = h.jc;
     S.rownorm = (double*)v;
     S.sig = (double*)(v + 8 * h.r_pad);
     S.scale_y = (double*)(v + 16 * h.r_pad);
@@ -1048,11 +1055,11 @@ int run_kraus_wave(hipmps_state_s* s) {
     S.keep_rule = 1;
     JacTask& J = hj[i];
     J.Y = X;
-    J.ldy = (int)h.c;
-    J.r = h.rows;
+    J.ldy = h.ld;
+    J.r = h.jr;
     J.r_pad = h.r_pad;
-    J.c = (int)h.c;
-    J.c_tot = (int)h.c;
+    J.c = h.jc;
+    J.c_tot = h.ld;
     J.nb = h.r_pad / JB;
     J.counter = d_counts + 2 * i;
     J.visited = d_counts + 2 * i + 1;
@@ -1067,8 +1074,12 @@ int run_kraus_wave(hipmps_state_s* s) {
   for (int i = 0; i < C; ++i) {
     const Ch& h = ch[i];
     HIP_TRY(hipMemcpyAsync(ws + h.off_k, hk + ko, (size_t)h.n_ops * 4 * sizeof(double2), hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_kraus_expand(s->sites[h.q].p, s->fp32, h.cl, h.kap, h.cr, (const double2*)(ws + h.off_k),
-                                h.n_ops, (double2*)(ws + h.off_x), h.r_pad, st));
+    if (h.adj)
+      HIP_TRY(launch_kraus_expand_adj(s->sites[h.q].p, s->fp32, h.cl, h.kap, h.cr, (const double2*)(ws + h.off_k),
+                                      h.n_ops, (double2*)(ws + h.off_x), h.ld, h.r_pad, st));
+    else
+      HIP_TRY(launch_kraus_expand(s->sites[h.q].p, s->fp32, h.cl, h.kap, h.cr, (const double2*)(ws + h.off_k),
+                                  h.n_ops, (double2*)(ws + h.off_x), h.r_pad, st));
     ko += 4 * (size_t)h.n_ops;
   }
   // sweeps over the channels not yet converged, one launch per round for all of them
@@ -1102,12 +1113,17 @@ int run_kraus_wave(hipmps_state_s* s) {
   for (int i = 0; i < C; ++i) {
     const Ch& h = ch[i];
     const int Kn = hcount[2 * C + i];
-    if (Kn < 1 || Kn > h.rows) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
+    if (Kn < 1 || Kn > h.jr) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
     const double* w = hw + (size_t)i * WS_N;
     if (w[5] != 0.0 || !std::isfinite(w[0] + w[1])) s->status = HIPMPS_E_NUMERIC;
     RC_TRY(ensure_site(s, h.q, (size_t)h.cl * 2 * Kn * h.cr, false, nullptr));
-    HIP_TRY(launch_kraus_gather((double2*)(ws + h.off_x), (int)h.c, (const int*)(ws + h.off_v + 40 * (size_t)h.r_pad),
-                                d_K + i, Kn, s->sites[h.q].p, s->fp32, h.cl, h.cr, st));
+    const int* perm = (const int*)(ws + h.off_v + 40 * (size_t)h.r_pad);
+    if (h.adj)
+      HIP_TRY(launch_kraus_gather_adj((double2*)(ws + h.off_x), h.ld, h.rows, (const double*)(ws + h.off_v), perm,
+                                      d_K + i, Kn, s->sites[h.q].p, s->fp32, h.cl, h.cr, st));
+    else
+      HIP_TRY(launch_kraus_gather((double2*)(ws + h.off_x), h.ld, perm, d_K + i, Kn, s->sites[h.q].p, s->fp32,
+                                  h.cl, h.cr, st));
     s->kap[h.q] = Kn;
     s->kraus_ops++;
   }

@@ -184,6 +184,13 @@ hipError_t launch_swap_mid(const void* src, void* dst, int L, int A, int B, int 
 hipError_t launch_kraus_expand(const void* site, int fp32, int cl, int kap, int cr, const double2* d_kraus,
                                int n_ops, double2* X, int rows_pad, hipStream_t s);
 // site[l][p][j][r] = Y[perm[j]][(l * 2 + p) * cr + r] for j < *K (at most k_max).
+// the adjoint layout for channels with more rows than columns (rows = kap n_ops, c = 2 cl cr):
+// Y (rows_pad x ldy, ldy = rows + c) = [X^H | I_c], pad rows zero; after the Jacobi, Kraus row j =
+// rownorm[perm[j]] * Y[perm[j]][off + col] (off = rows)
+hipError_t launch_kraus_expand_adj(const void* site, int fp32, int cl, int kap, int cr, const double2* d_kraus,
+                                   int n_ops, double2* Y, int ldy, int rows_pad, hipStream_t s);
+hipError_t launch_kraus_gather_adj(const double2* Y, int ldy, int off, const double* rownorm, const int* perm,
+                                   const int* K, int k_max, void* site, int fp32, int cl, int cr, hipStream_t s);
 hipError_t launch_kraus_gather(const double2* Y, int ldy, const int* perm, const int* K, int k_max, void* site,
                                int fp32, int cl, int cr, hipStream_t s);
 
