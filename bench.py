@@ -331,9 +331,17 @@ def main():
                     help="Jacobi on the double-LQ factor for min(m,n) >= this (0 off; default: engine default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (profiler counter calibration)")
+    ap.add_argument("--noisy-only", action="store_true",
+                    help="only the noisy block (for profiling the purified-MPS path)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.noisy_only:
+        eng = mps.Engine()
+        nt, _, kd, bonds = noisy_gpu(eng, noisy_workload())
+        eng.close()
+        print(json.dumps({"noisy_ms_per_circuit": round(nt * 1e3, 2), "max_kraus": kd, "max_bond": bonds}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); independent circuits shard "
