@@ -315,3 +315,19 @@ def test_channels_queued_equal_one_at_a_time(eng):
     assert abs(a.norm() - b.norm()) < 1e-12
     a.close()
     b.close()
+
+
+def test_reset_drops_queued_channels(eng):
+    """Channels still queued when a state is reset are dropped with it: |0..0>, Kraus legs 1."""
+    say("reset with queued channels")
+    s = mps.MpsState(eng, 4, cutoff=1e-9)
+    s.run([{"name": "H", "qubits": (0,)}, {"name": "CNOT", "qubits": (0, 1)}])
+    s.apply_kraus(0, N.amplitude_damping(0.3))
+    s.apply_kraus(1, N.depolarizing(0.2))
+    s.reset()
+    assert list(s.kraus_dims()) == [1, 1, 1, 1]
+    assert np.abs(s.expval_z_all() - 1.0).max() < 1e-14
+    s.run([{"name": "X", "qubits": (2,)}])
+    s.apply_kraus(2, N.amplitude_damping(0.25))
+    assert abs(s.expval_z_all()[2] - (-1 + 2 * 0.25)) < 1e-12
+    s.close()

@@ -362,6 +362,7 @@ int state_init_product(hipmps_state_s* s) {
   s->busy.assign(s->n, 0);
   s->cbusy.assign(s->n, 0);
   s->wave.clear();
+  s->cwave.clear();
   s->discarded = 0.0;
   s->fid = 1.0;
   s->maxb = 1;
