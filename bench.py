@@ -24,7 +24,7 @@ Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
   noisy              the purified-MPS (noisy) path: a 40-qubit u3/CNOT circuit of depth 4 with the
                      ibmqx2 backend's single-qubit channels after every gate (qubit q takes the
                      channels of backend qubit q mod 5), no bond cap, cut 1e-9: wall time per
-                     circuit with its <Z> read back
+                     circuit with its <Z> read back; batch_64, 64 copies advanced together
   cpu_baseline       the FP64 CPU oracle on the host cores at chi = 256; under "noisy", the
                      restated purified-MPS algorithm on the same noisy circuit, with the largest
                      <Z> difference from the GPU result
@@ -267,6 +267,24 @@ def noisy_gpu(eng, ops, reps=5):
     return float(np.median(times[1:])), ez, kd, bonds
 
 
+def noisy_batch(eng, ops, count=64):
+    """The same noisy circuit `count` times, advanced together (mps.run_noisy_batch: the open
+    waves of all of them run as one gate wave and one channel wave); wall time and circuits/s."""
+    chans = {id(g): c for g, c in ops}
+
+    class Model:
+        def channels(self, g):
+            return chans[id(g)]
+
+    gates = [g for g, _ in ops]
+    mps.run_noisy_batch(eng, [(NOISY_N, gates)] * 2, Model())  # warm the arena at batch shapes
+    t0 = time.perf_counter()
+    res = mps.run_noisy_batch(eng, [(NOISY_N, gates)] * count, Model())
+    dt = time.perf_counter() - t0
+    spread = max(float(np.abs(r["expval_z"] - res[0]["expval_z"]).max()) for r in res)
+    return dt, spread
+
+
 def cpu_baseline_noisy(ops, ez_gpu, seconds=3.0):
     """The restated purified-MPS algorithm (oracle/noise.py, numpy) on the same noisy circuit."""
     from threadpoolctl import threadpool_limits
@@ -496,6 +514,10 @@ def main():
                                     "channels after every gate (backend qubit q mod 5), cut 1e-9, no bond cap",
                         "gates": len(noisy_ops), "channels": nch, "ms_per_circuit": round(nt * 1e3, 2),
                         "circuits_per_s": round(1.0 / nt, 3), "max_kraus": kd, "max_bond": bonds}
+        bt, spread = noisy_batch(eng, noisy_ops)
+        log(f"noisy batch: 64 circuits together, {bt:.2f} s, {64 / bt:.1f} circuits/s")
+        out["noisy"]["batch_64"] = {"seconds": round(bt, 3), "circuits_per_s": round(64 / bt, 2),
+                                    "max_abs_dz_across_copies": spread}
     eng.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (FP64 oracle)")
@@ -503,7 +525,9 @@ def main():
         if noisy_ops is not None:
             log("cpu baseline, noisy (purified-MPS oracle)")
             out["cpu_baseline"]["noisy"] = cpu_baseline_noisy(noisy_ops, noisy_ez)
-            out["noisy"]["speedup_vs_cpu"] = round(out["noisy"]["circuits_per_s"] / out["cpu_baseline"]["noisy"]["value"], 2)
+            cpu_rate = out["cpu_baseline"]["noisy"]["value"]
+            out["noisy"]["speedup_vs_cpu"] = round(out["noisy"]["circuits_per_s"] / cpu_rate, 2)
+            out["noisy"]["batch_64"]["speedup_vs_cpu"] = round(out["noisy"]["batch_64"]["circuits_per_s"] / cpu_rate, 2)
     print(json.dumps(out), flush=True)
 
 

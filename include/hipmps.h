@@ -166,6 +166,16 @@ int hipmps_set_site(hipmps_state s, int k, int chi_l, int chi_r, const double* i
 #define HIPMPS_MAX_KRAUS_OPS 16
 int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats);
 int hipmps_kraus_dims(hipmps_state s, int* out_n); /* kap_k per site */
+/* Drivers of many states at once (no counterpart in the reference, whose visitor runs one
+ * circuit).  hipmps_queue_gates queues gates up to the first 2-qubit gate that would have to
+ * wait for the open waves (it shares a site with a queued gate or channel) and reports how many
+ * it queued; hipmps_queue_kraus queues a channel unless one is already queued on the site
+ * (*queued = 0).  hipmps_run_waves then runs the open waves of all the given states (one
+ * engine, each listed once) together: their gates as one wave, then their channels as one, after
+ * which the waiting operation can be queued.  Results equal those of the states run alone. */
+int hipmps_queue_gates(hipmps_state s, int n, const hipmps_gate* gates, int* n_queued);
+int hipmps_queue_kraus(hipmps_state s, int q, int n_ops, const double* mats, int* queued);
+int hipmps_run_waves(hipmps_state* states, int n);
 /* Reduced density matrix of k qubits (1 <= k <= 15, the reference's limit), the other qubits
  * traced out; with k = n the full density matrix.  Replaces the "density_matrix" contraction of
  * ExaTnPmpsVisitor::finalize (ExaTnPmpsVisitor.cpp:657-761).  HIPMPS_E_NOMEM when its device

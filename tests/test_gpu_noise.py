@@ -331,3 +331,52 @@ def test_reset_drops_queued_channels(eng):
     s.apply_kraus(2, N.amplitude_damping(0.25))
     assert abs(s.expval_z_all()[2] - (-1 + 2 * 0.25)) < 1e-12
     s.close()
+
+
+class _EveryGate:
+    """A channel list per gate: the given channels on each qubit the gate touches."""
+
+    def __init__(self, chans):
+        self.chans = chans
+
+    def channels(self, g):
+        return [(q, self.chans[q % len(self.chans)]) for q in g["qubits"]]
+
+
+def test_noisy_batch_equals_alone(eng):
+    """Noisy circuits of different sizes advanced together (hipmps_queue_gates / queue_kraus /
+    run_waves) give what each gives alone: the joint waves only share launches."""
+    say("noisy batch vs alone")
+    noise = _EveryGate([N.amplitude_damping(0.07), N.depolarizing(0.1), N.phase_damping(0.2)])
+    circuits = [(n, C.u3_cnot_line(n, d, seed=100 + n + d)) for n, d in ((5, 3), (8, 4), (3, 6), (12, 2), (6, 5))]
+    got = mps.run_noisy_batch(eng, circuits, noise)
+    for (n, gates), r in zip(circuits, got):
+        s = mps.MpsState(eng, n, cutoff=1e-9)
+        for g in gates:
+            s.run([g])
+            for q, k in noise.channels(g):
+                s.apply_kraus(q, k)
+        assert list(r["kraus_dims"]) == list(s.kraus_dims())
+        assert list(r["bonds"]) == list(s.bonds())
+        assert np.abs(r["expval_z"] - s.expval_z_all()).max() < 1e-12
+        s.close()
+    # against the restated reference algorithm (same 2 kap bound) for one of them
+    n, gates = circuits[2]
+    ref = PurifiedMps(n)
+    for g in gates:
+        ref.gate(mps.matrix_of(g), g["qubits"])
+        for q, k in noise.channels(g):
+            ref.channel(q, k)
+    assert list(got[2]["kraus_dims"]) == ref.kraus_dims()
+    assert np.abs(got[2]["expval_z"] - ref.expval_z()).max() < 1e-8
+    # argument checks: a state listed twice, states of two engines
+    a = mps.MpsState(eng, 2)
+    with pytest.raises(capi.HipMpsError):
+        mps.run_waves([a, a])
+    other = mps.Engine()
+    b = mps.MpsState(other, 2)
+    with pytest.raises(capi.HipMpsError):
+        mps.run_waves([a, b])
+    b.close()
+    other.close()
+    a.close()

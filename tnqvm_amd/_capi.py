@@ -37,6 +37,7 @@ EXPORTS = [
     "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing", "hipmps_engine_set_lq_precondition",
     "hipmps_expval_diag", "hipmps_state_clone", "hipmps_apply_kraus", "hipmps_kraus_dims",
     "hipmps_reduced_density_matrix", "hipmps_engine_get_tuning", "hipmps_engine_set_tuning",
+    "hipmps_queue_gates", "hipmps_queue_kraus", "hipmps_run_waves",
 ]
 
 
@@ -150,6 +151,9 @@ def lib():
         "hipmps_reduced_density_matrix": ([vp, pint, ip, dp, ctypes.c_size_t], ip),
         "hipmps_engine_get_tuning": ([vp, ctypes.POINTER(Tuning)], ip),
         "hipmps_engine_set_tuning": ([vp, ctypes.POINTER(Tuning)], ip),
+        "hipmps_queue_gates": ([vp, ip, ctypes.POINTER(Gate), pint], ip),
+        "hipmps_queue_kraus": ([vp, ip, ip, dp, pint], ip),
+        "hipmps_run_waves": ([ctypes.POINTER(vp), ip], ip),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

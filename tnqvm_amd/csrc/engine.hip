@@ -864,7 +864,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
 // ------------------------------------------------------------------------------------
 // per-state gate queue
 // ------------------------------------------------------------------------------------
-int run_kraus_wave(hipmps_state_s* s);
+int run_kraus_waves(hipmps_engine E, const std::vector<hipmps_state_s*>& states);
 
 // the open wave of gates, then the channels queued after it
 int state_run_wave(hipmps_state_s* s) {
@@ -883,7 +883,7 @@ int state_run_wave(hipmps_state_s* s) {
     std::fill(s->busy.begin(), s->busy.end(), 0);
     if (rc != 0) return rc;
   }
-  return run_kraus_wave(s);
+  return run_kraus_waves(s->e, {s});
 }
 
 // Queue a 2-qubit gate; returns 1 if the gate conflicts with the open wave (caller runs
@@ -967,14 +967,18 @@ int state_flush(hipmps_state_s* s) {
 // the next wave.
 constexpr double KRAUS_EPS = 1e-9;
 
-int run_kraus_wave(hipmps_state_s* s) {
-  if (s->cwave.empty()) return 0;
-  hipmps_engine E = s->e;
+int run_kraus_waves(hipmps_engine E, const std::vector<hipmps_state_s*>& states) {
+  std::vector<std::pair<hipmps_state_s*, const hipmps_state_s::QK*>> list;
+  for (auto* st_ : states)
+    for (const auto& k : st_->cwave) list.push_back({st_, &k});
+  if (list.empty()) return 0;
   const hipStream_t st = E->stream;
-  const int C = (int)s->cwave.size();
+  const int C = (int)list.size();
   // adj: more rows (kap n_ops) than columns (2 cl cr); the Jacobi then runs on the rows of
   // [X^H | I] (jr = c rows, ld = rows + c), a sweep of few rounds instead of many
   struct Ch {
+    hipmps_state_s* s;
+    const hipmps_state_s::QK* k;
     int q, cl, cr, kap, n_ops, rows, r_pad, jr, jc, ld;
     bool adj;
     size_t c, off_x, off_v, off_k;
@@ -984,11 +988,13 @@ int run_kraus_wave(hipmps_state_s* s) {
   int max_rows = 1;
   for (int i = 0; i < C; ++i) {
     Ch& h = ch[i];
-    h.q = s->cwave[i].q;
-    h.n_ops = s->cwave[i].n_ops;
-    h.cl = s->chi[h.q];
-    h.cr = s->chi[h.q + 1];
-    h.kap = s->kap[h.q];
+    h.s = list[i].first;
+    h.k = list[i].second;
+    h.q = h.k->q;
+    h.n_ops = h.k->n_ops;
+    h.cl = h.s->chi[h.q];
+    h.cr = h.s->chi[h.q + 1];
+    h.kap = h.s->kap[h.q];
     h.rows = h.kap * h.n_ops;
     if (h.rows > 4096) return fail(HIPMPS_E_UNSUPPORTED, "Kraus leg above 4096 rows");
     h.c = (size_t)2 * h.cl * h.cr;
@@ -1030,8 +1036,8 @@ int run_kraus_wave(hipmps_state_s* s) {
   size_t ko = 0;
   for (int i = 0; i < C; ++i) {
     const Ch& h = ch[i];
-    for (int t = 0; t < 4 * h.n_ops; ++t)
-      hk[ko + t] = make_double2(s->cwave[i].K[t].real(), s->cwave[i].K[t].imag());
+    hipmps_state_s* s = h.s;
+    for (int t = 0; t < 4 * h.n_ops; ++t) hk[ko + t] = make_double2(h.k->K[t].real(), h.k->K[t].imag());
     char* v = ws + h.off_v;
     double2* X = (double2*)(ws + h.off_x);
     SplitTask& S = hs[i];
@@ -1074,6 +1080,7 @@ int run_kraus_wave(hipmps_state_s* s) {
   ko = 0;
   for (int i = 0; i < C; ++i) {
     const Ch& h = ch[i];
+    hipmps_state_s* s = h.s;
     HIP_TRY(hipMemcpyAsync(ws + h.off_k, hk + ko, (size_t)h.n_ops * 4 * sizeof(double2), hipMemcpyHostToDevice, st));
     if (h.adj)
       HIP_TRY(launch_kraus_expand_adj(s->sites[h.q].p, s->fp32, h.cl, h.kap, h.cr, (const double2*)(ws + h.off_k),
@@ -1105,7 +1112,7 @@ int run_kraus_wave(hipmps_state_s* s) {
       if (hcount[2 * i] != 0) next.push_back(i);
     active.swap(next);
   }
-  if (!active.empty()) s->status = HIPMPS_E_CONVERGENCE;
+  for (int i : active) ch[i].s->status = HIPMPS_E_CONVERGENCE;
   HIP_TRY(launch_rownorm(dsp, C, max_rows, st));
   HIP_TRY(launch_select(dsp, C, st));
   HIP_TRY(hipMemcpyAsync(hcount + 2 * C, d_K, (size_t)C * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1113,6 +1120,7 @@ int run_kraus_wave(hipmps_state_s* s) {
   HIP_TRY(hipStreamSynchronize(st));
   for (int i = 0; i < C; ++i) {
     const Ch& h = ch[i];
+    hipmps_state_s* s = h.s;
     const int Kn = hcount[2 * C + i];
     if (Kn < 1 || Kn > h.jr) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
     const double* w = hw + (size_t)i * WS_N;
@@ -1129,8 +1137,10 @@ int run_kraus_wave(hipmps_state_s* s) {
     s->kraus_ops++;
   }
   HIP_TRY(hipStreamSynchronize(st));
-  s->cwave.clear();
-  std::fill(s->cbusy.begin(), s->cbusy.end(), 0);
+  for (auto* st_ : states) {
+    st_->cwave.clear();
+    std::fill(st_->cbusy.begin(), st_->cbusy.end(), 0);
+  }
   return 0;
 }
 
@@ -2098,6 +2108,61 @@ int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats) {
   s->cwave.push_back(ch);
   s->cbusy[q] = 1;
   return 0;
+}
+
+int hipmps_queue_gates(hipmps_state s, int n, const hipmps_gate* gates, int* n_queued) {
+  if (!s || !n_queued || (n > 0 && !gates)) return fail(HIPMPS_E_ARG, "null argument");
+  *n_queued = 0;
+  RC_TRY(set_device(s->e));
+  for (int k = 0; k < n; ++k) {
+    const hipmps_gate& g = gates[k];
+    if (g.nq == 2 && g.q0 >= 0 && g.q0 < s->n && g.q1 >= 0 && g.q1 < s->n && std::abs(g.q0 - g.q1) == 1) {
+      const int i = std::min(g.q0, g.q1);
+      if (s->busy[i] || s->busy[i + 1] || s->cbusy[i] || s->cbusy[i + 1]) return 0;  // after the open waves
+    }
+    RC_TRY(apply_gate_list(s, 1, &g));
+    ++*n_queued;
+  }
+  return 0;
+}
+
+int hipmps_queue_kraus(hipmps_state s, int q, int n_ops, const double* mats, int* queued) {
+  if (!s || !queued || q < 0 || q >= s->n) return fail(HIPMPS_E_ARG, "bad arguments");
+  *queued = 0;
+  if (s->cbusy[q]) return 0;  // one queued channel per site
+  RC_TRY(hipmps_apply_kraus(s, q, n_ops, mats));
+  *queued = 1;
+  return 0;
+}
+
+int hipmps_run_waves(hipmps_state* states, int n) {
+  if (n < 0 || (n > 0 && !states)) return fail(HIPMPS_E_ARG, "bad arguments");
+  if (n == 0) return 0;
+  hipmps_engine E = states[0] ? states[0]->e : nullptr;
+  for (int k = 0; k < n; ++k) {
+    if (!states[k]) return fail(HIPMPS_E_ARG, "null state");
+    if (states[k]->e != E) return fail(HIPMPS_E_ARG, "states of different engines");
+    for (int j = 0; j < k; ++j)
+      if (states[j] == states[k]) return fail(HIPMPS_E_ARG, "a state is listed twice");
+  }
+  RC_TRY(set_device(E));
+  std::vector<WaveItem> items;
+  std::vector<hipmps_state_s*> list(states, states + n);
+  for (auto* st : list)
+    for (auto& q : st->wave) {
+      WaveItem it{};
+      it.s = st;
+      it.i = q.i;
+      it.G = q.G;
+      items.push_back(it);
+    }
+  int rc = items.empty() ? 0 : run_wave(E, items);
+  for (auto* st : list) {
+    st->wave.clear();
+    std::fill(st->busy.begin(), st->busy.end(), 0);
+  }
+  if (rc != 0) return rc;
+  return run_kraus_waves(E, list);
 }
 
 int hipmps_kraus_dims(hipmps_state s, int* out) {

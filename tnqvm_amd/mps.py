@@ -311,6 +311,21 @@ class MpsState:
         check(lib().hipmps_apply_kraus(self.h, int(q), len(k), k.view(np.float64).ctypes.data_as(_dp)))
         return self
 
+    def queue_gates(self, arr, start: int, count: int) -> int:
+        """Queue gates arr[start:start + count] (a to_c_gates array) up to the first 2-qubit gate
+        that must wait for the open waves; returns how many were queued."""
+        nq = ctypes.c_int(0)
+        ptr = ctypes.cast(ctypes.addressof(arr) + start * ctypes.sizeof(C.Gate), ctypes.POINTER(C.Gate))
+        check(lib().hipmps_queue_gates(self.h, count, ptr, ctypes.byref(nq)))
+        return nq.value
+
+    def queue_kraus(self, q: int, mats: np.ndarray) -> bool:
+        """Queue a channel (mats: float64 view of n_ops 2 x 2 complex operators) unless one is
+        already queued on q."""
+        ok = ctypes.c_int(0)
+        check(lib().hipmps_queue_kraus(self.h, int(q), mats.size // 8, mats.ctypes.data_as(_dp), ctypes.byref(ok)))
+        return bool(ok.value)
+
     def clone(self) -> "MpsState":
         """Deep copy on the same engine (hipmps_state_clone)."""
         h = ctypes.c_void_p()
@@ -360,6 +375,59 @@ class MpsState:
     def set_site(self, k: int, a: np.ndarray):
         a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
         check(lib().hipmps_set_site(self.h, k, a.shape[0], a.shape[2], a.view(np.float64).ctypes.data_as(_dp)))
+
+
+def run_waves(states):
+    """Run the open waves of several states (one engine) together: gates, then channels."""
+    arr = (ctypes.c_void_p * len(states))(*[st.h for st in states])
+    check(lib().hipmps_run_waves(arr, len(states)))
+
+
+def run_noisy_batch(engine: Engine, circuits, noise, precision: str = "fp64", max_bond: int = 0):
+    """Many noisy circuits advanced together: each is fed up to the next operation that must wait
+    for its open waves, then the open waves of all of them run as one gate wave and one channel
+    wave.  circuits: [(n, gates)]; noise as in run_noisy.  Returns per circuit the same fields as
+    run_noisy's expval_z, bonds, kraus_dims and trace (equal to running each alone)."""
+    prepared = []
+    cache = {}
+    for n, gates in circuits:
+        gates = [g for g in gates if g["name"] not in ("Measure", "I")]
+        arr, _ = to_c_gates(gates)
+        ops = []
+        for k, g in enumerate(gates):
+            ops.append((k, None, None))
+            if noise is not None:
+                for q, kops in noise.channels(g):
+                    key = id(kops)
+                    if key not in cache:
+                        m = np.ascontiguousarray(np.asarray(kops, dtype=np.complex128).reshape(-1, 2, 2))
+                        cache[key] = (kops, m.view(np.float64).reshape(-1))
+                    ops.append((None, q, cache[key][1]))
+        prepared.append((MpsState(engine, n, cutoff=1e-9, max_bond=max_bond, precision=precision), arr, ops))
+    try:
+        cursor = [0] * len(prepared)
+        while True:
+            live = []
+            for i, (st, arr, ops) in enumerate(prepared):
+                c = cursor[i]
+                while c < len(ops):
+                    k, q, m = ops[c]
+                    if k is not None:
+                        if st.queue_gates(arr, k, 1) == 0:
+                            break
+                    elif not st.queue_kraus(q, m):
+                        break
+                    c += 1
+                cursor[i] = c
+                live.append(st)
+            run_waves(live)
+            if all(cursor[i] == len(p[2]) for i, p in enumerate(prepared)):
+                break
+        return [{"expval_z": st.expval_z_all(), "bonds": st.bonds(), "kraus_dims": st.kraus_dims(),
+                 "trace": st.norm()} for st, _, _ in prepared]
+    finally:
+        for st, _, _ in prepared:
+            st.close()
 
 
 def run_noisy(engine: Engine, n: int, gates, noise=None, measured=(), shots: int = 1024, seed=None,
