@@ -1,8 +1,9 @@
 """Noisy-simulation oracle: a dense density matrix, and a restatement of the reference's
 locally purified MPS (tnqvm/visitors/exatn-mpo/ExaTnPmpsVisitor.cpp), both in numpy.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product library
-(tnqvm_amd / libhipmps) never imports this module.
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker, and by the cpu_baseline leg of
+bench.py (PurifiedMps timed on the host as the noisy path's CPU baseline, and its <Z> compared
+with the device's).  The product library (tnqvm_amd / libhipmps) never imports this module.
 
 Conventions, as in the reference: qubit k is bit k of a basis index (LSB = qubit 0); the first
 operand of a 2-qubit gate is the most significant bit of its matrix index (Gates.hpp:237-246).
@@ -11,6 +12,8 @@ operand of a 2-qubit gate is the most significant bit of its matrix index (Gates
   Pinned by closed forms in tests/test_oracle_noise.py (amplitude damping of |1>, dephasing
   of |+>, depolarizing of a Bell pair, trace preservation, agreement with the pure-state
   oracle for unitary circuits).
+* PurifiedMps.expval_z: Tr(rho Z_q) through environments, pinned against the dense route
+  (as_density().z) in tests/test_oracle_noise.py.
 * PurifiedMps: sites Q_k[l][p][k][r] (bond, physical, Kraus, bond), following the
   reference's steps:
     - initial |0..0>, all legs of extent 1 (buildInitialNetwork, :501-531);
