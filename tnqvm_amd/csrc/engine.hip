@@ -466,7 +466,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   // descriptors
   size_t desc_bytes = align_up(N * sizeof(ThetaTask)) + align_up(N * sizeof(SplitTask)) +
                       align_up(N * sizeof(JacTask)) + align_up(N * sizeof(GemmTask)) +
-                      align_up(2 * N * sizeof(LqTask)) + align_up(3 * N * sizeof(TriTask));
+                      align_up(2 * N * sizeof(LqTask)) + align_up(3 * N * sizeof(TriTask)) +
+                      align_up(N * sizeof(double));
   RC_TRY(E->desc.ensure(desc_bytes));
   // host staging: descriptors | counts (2N ints) | active Jacobi list / results
   const size_t stage_bytes = desc_bytes + align_up(2 * N * sizeof(int)) + N * sizeof(JacTask) + WS_N * N * sizeof(double) + 4096;
@@ -478,6 +479,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   GemmTask* hgm = (GemmTask*)((char*)hjt + align_up(N * sizeof(JacTask)));
   LqTask* hlq = (LqTask*)((char*)hgm + align_up(N * sizeof(GemmTask)));
   TriTask* htr = (TriTask*)((char*)hlq + align_up(2 * N * sizeof(LqTask)));
+  double* hcut = (double*)((char*)htr + align_up(3 * N * sizeof(TriTask)));  // rank cut, per LQ item
   char* dd = (char*)E->desc.p;
   ThetaTask* dth = (ThetaTask*)dd;
   SplitTask* dsp = (SplitTask*)(dd + align_up(N * sizeof(ThetaTask)));
@@ -485,6 +487,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   GemmTask* dgm = (GemmTask*)((char*)djt + align_up(N * sizeof(JacTask)));
   LqTask* dlq = (LqTask*)((char*)dgm + align_up(N * sizeof(GemmTask)));
   TriTask* dtr = (TriTask*)((char*)dlq + align_up(2 * N * sizeof(LqTask)));
+  const double* dcut = (const double*)((char*)dtr + align_up(3 * N * sizeof(TriTask)));
 
   // group boundaries
   int n64 = 0;
@@ -581,6 +584,12 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     htr[nlq] = TriTask{Y, Z, it.c, it.r, it.r, it.r};               // Z = L^H
     htr[N + nlq] = TriTask{Z, Jm, it.r, it.r, it.r_pad, it.r};      // J = L_Z^H, pad rows zero
     htr[2 * N + nlq] = TriTask{Jm, Yt, it.r, it.r, it.r_pad, it.c};  // Yt = [J | 0]
+    // the singular value below which the truncation drops an index (partial norm sigma, or
+    // sqrt(sigma) with the 2-norm definition); the rank cut keeps its perturbation far below
+    {
+      const double cutoff = it.s->t.svd_cutoff;
+      hcut[nlq] = it.s->t.partial_norm == HIPMPS_PN_L2 ? cutoff * cutoff : cutoff;
+    }
     ++nlq;
     lq_max_r = std::max(lq_max_r, it.r);
     lq_max_rpad = std::max(lq_max_rpad, it.r_pad);
@@ -669,6 +678,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       HIP_TRY(launch_lower_to_upper_h(dtr + N + b, n, lq_max_r, lq_max_rpad, ls));
     }
     RC_TRY(join_lanes(lq_lanes));
+    HIP_TRY(launch_rank_cut(dtr + N, dcut, nlq, st));
     for (int q = 0; q < N; ++q) {
       auto& it = items[q];
       if (!it.lq) continue;

@@ -118,6 +118,10 @@ struct TriTask {
   int ld_src, n, rows_dst, ld_dst;
 };
 
+// zero the rows of the n x n factor tasks[t].dst whose norm is below 1e-14 of its Frobenius norm,
+// where that moves no singular value by more than 1e-3 sigma_cut[t] (<= 0: skip); n <= 4096
+hipError_t launch_rank_cut(const TriTask* d_tasks, const double* d_sigma_cut, int ntasks, hipStream_t s);
+
 // Chain step for amplitudes / slices: rows of partial contractions v[row][l].
 struct ChainTask {
   const double2* vin;   // rows_in x cl

@@ -346,4 +346,51 @@ hipError_t launch_lower_to_upper_h(const TriTask* d_tasks, int ntasks, int max_n
   return hipGetLastError();
 }
 
+// Rank cut of the preconditioned factor J (n x n, ld_dst; TriTask.dst, .n): rows whose squared
+// norm is below 1e-28 ||J||_F^2 are set to zero before the sweeps.  Zeroing them perturbs J by
+// at most 1e-14 sqrt(n) ||J||_F, which bounds the change of every singular value (Weyl); the cut
+// is made only where that bound is below 1e-3 of sigma_cut[t], the singular value under which
+// the truncation drops an index, so neither a kept value nor the bond dimension moves.  Rows of
+// rounding noise (a rank-deficient theta', as on purified sites) then no longer keep the
+// relative convergence test rotating; a full-rank J is only read.
+__global__ __launch_bounds__(256) void k_rank_cut(const TriTask* tasks, const double* sigma_cut) {
+  __shared__ double rn[4096];
+  __shared__ double red[4];
+  __shared__ double s_thr;
+  const TriTask& T = tasks[blockIdx.x];
+  const double cut = sigma_cut[blockIdx.x];
+  if (!(cut > 0.0)) return;
+  const int n = T.n;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  double part = 0.0;
+  for (int i = wave; i < n; i += 4) {
+    const double2* row = T.dst + (size_t)i * T.ld_dst;
+    double a = 0.0;
+    for (int j = lane; j < n; j += 64) a += cabs2(row[j]);
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if (lane == 0) rn[i] = a;
+    part += a;
+  }
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double fro2 = red[0] + red[1] + red[2] + red[3];
+    s_thr = (1e-14 * sqrt((double)n * fro2) <= 1e-3 * cut) ? 1e-28 * fro2 : -1.0;
+  }
+  __syncthreads();
+  const double thr = s_thr;
+  if (thr < 0.0) return;
+  for (int i = wave; i < n; i += 4) {
+    if (rn[i] >= thr) continue;
+    double2* row = T.dst + (size_t)i * T.ld_dst;
+    for (int j = lane; j < n; j += 64) row[j] = make_double2(0.0, 0.0);
+  }
+}
+
+hipError_t launch_rank_cut(const TriTask* d_tasks, const double* d_sigma_cut, int ntasks, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rank_cut, dim3(ntasks), dim3(256), 0, s, d_tasks, d_sigma_cut);
+  return hipGetLastError();
+}
+
 }  // namespace hipmps
