@@ -1206,8 +1206,49 @@ struct EnvJob {
   const void* Abra = nullptr;  // bra-side site when it differs from A (projected copies)
 };
 
-// left step: E'(cr x cr) = A^H D (E A), D = diag(+-1) on the physical index when z
-int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
+// left step: E'(cr x cr) = A^H D (E A), D = diag(+-1) on the physical index when z; the two
+// GEMMs of one step (tile_begin 0)
+void env_tasks_left(const EnvJob& j, GemmTask& a, GemmTask& b) {
+  memset(&a, 0, sizeof(a));
+  const int d = 2 * j.kap;  // physical index and Kraus leg, (p, k) with k fastest
+  a.A = j.Ein; a.lda = j.cl;
+  a.B = j.A; a.ldb = d * j.cr;
+  a.C = j.X; a.ldc = d * j.cr;
+  a.M = j.cl; a.N = d * j.cr; a.K = j.cl;
+  a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
+  memset(&b, 0, sizeof(b));
+  b.A = j.Abra ? j.Abra : j.A; b.lda = j.cr;
+  b.B = j.X; b.ldb = j.cr;
+  b.C = j.Eout; b.ldc = j.cr;
+  b.M = j.cr; b.N = j.cr; b.K = d * j.cl;
+  b.sign_div = j.z ? j.kap : 0;
+  b.kd0 = j.d0;
+  b.kd1 = j.d1;
+  b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
+}
+
+// right step: E'(cl x cl) = conj(A) (A R^T)^T with the physical sign when z
+void env_tasks_right(const EnvJob& j, GemmTask& a, GemmTask& b) {
+  memset(&a, 0, sizeof(a));
+  const int d = 2 * j.kap;
+  a.A = j.A; a.lda = j.cr;                 // (d cl) x cr
+  a.B = j.Ein; a.ldb = j.cr;               // R^T
+  a.C = j.X; a.ldc = j.cr;
+  a.M = d * j.cl; a.N = j.cr; a.K = j.cr;
+  a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
+  memset(&b, 0, sizeof(b));
+  b.A = j.A; b.lda = d * j.cr;             // conj, cl x (d cr)
+  b.B = j.X; b.ldb = d * j.cr;             // X viewed cl x (d cr), transposed
+  b.C = j.Eout; b.ldc = j.cl;
+  b.M = j.cl; b.N = j.cl; b.K = d * j.cr;
+  b.sign_div = j.z ? j.kap * j.cr : 0;
+  b.kd0 = j.d0;
+  b.kd1 = j.d1;
+  b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
+}
+
+// independent steps, one launch per GEMM for all of them
+int env_steps(hipmps_engine E, std::vector<EnvJob>& jobs, bool left, int fp32) {
   if (jobs.empty()) return 0;
   const int N = (int)jobs.size();
   RC_TRY(E->hdesc.ensure(2 * N * sizeof(GemmTask)));
@@ -1215,72 +1256,57 @@ int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
   GemmTask* h = (GemmTask*)E->hdesc.p;
   int t1 = 0, t2 = 0;
   for (int q = 0; q < N; ++q) {
-    auto& j = jobs[q];
     GemmTask& a = h[q];
-    memset(&a, 0, sizeof(a));
-    const int d = 2 * j.kap;  // physical index and Kraus leg, (p, k) with k fastest
-    a.A = j.Ein; a.lda = j.cl;
-    a.B = j.A; a.ldb = d * j.cr;
-    a.C = j.X; a.ldc = d * j.cr;
-    a.M = j.cl; a.N = d * j.cr; a.K = j.cl;
-    a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
-    a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
     GemmTask& b = h[N + q];
-    memset(&b, 0, sizeof(b));
-    b.A = j.Abra ? j.Abra : j.A; b.lda = j.cr;
-    b.B = j.X; b.ldb = j.cr;
-    b.C = j.Eout; b.ldc = j.cr;
-    b.M = j.cr; b.N = j.cr; b.K = d * j.cl;
-    b.sign_div = j.z ? j.kap : 0;
-    b.kd0 = j.d0;
-    b.kd1 = j.d1;
-    b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
+    if (left) env_tasks_left(jobs[q], a, b);
+    else env_tasks_right(jobs[q], a, b);
+    a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
     b.tile_begin = t2; t2 += b.tiles_m * b.tiles_n;
   }
   GemmTask* d = (GemmTask*)E->desc.p;
   HIP_TRY(hipMemcpyAsync(d, h, 2 * N * sizeof(GemmTask), hipMemcpyHostToDevice, E->stream));
-  HIP_TRY(launch_gemm(d, N, t1, 0, 0, 0, fp32, 0, E->stream));
-  HIP_TRY(launch_gemm(d + N, N, t2, 2, 0, fp32, 0, 0, E->stream));
+  if (left) {
+    HIP_TRY(launch_gemm(d, N, t1, 0, 0, 0, fp32, 0, E->stream));
+    HIP_TRY(launch_gemm(d + N, N, t2, 2, 0, fp32, 0, 0, E->stream));
+  } else {
+    HIP_TRY(launch_gemm(d, N, t1, 0, 3, fp32, 0, 0, E->stream));
+    HIP_TRY(launch_gemm(d + N, N, t2, 1, 3, fp32, 0, 0, E->stream));
+  }
   HIP_TRY(hipStreamSynchronize(E->stream));  // staging reuse
   return 0;
 }
 
-// right step: E'(cl x cl) = conj(A) (A R^T)^T with the physical sign when z
-int env_right(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) {
+int env_left(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) { return env_steps(E, jobs, true, fp32); }
+int env_right(hipmps_engine E, std::vector<EnvJob>& jobs, int fp32) { return env_steps(E, jobs, false, fp32); }
+
+// a chain of steps, each reading the environment the one before wrote (a sweep over the
+// sites): every descriptor is staged at once and the steps launch back to back in stream
+// order (the X scratch they share is rewritten only after the step before has read it), with
+// one synchronisation at the end instead of one per site
+int env_chain(hipmps_engine E, const std::vector<EnvJob>& jobs, bool left, int fp32) {
   if (jobs.empty()) return 0;
   const int N = (int)jobs.size();
   RC_TRY(E->hdesc.ensure(2 * N * sizeof(GemmTask)));
   RC_TRY(E->desc.ensure(2 * N * sizeof(GemmTask)));
   GemmTask* h = (GemmTask*)E->hdesc.p;
-  int t1 = 0, t2 = 0;
   for (int q = 0; q < N; ++q) {
-    auto& j = jobs[q];
-    GemmTask& a = h[q];
-    memset(&a, 0, sizeof(a));
-    const int d = 2 * j.kap;
-    a.A = j.A; a.lda = j.cr;                 // (d cl) x cr
-    a.B = j.Ein; a.ldb = j.cr;               // R^T
-    a.C = j.X; a.ldc = j.cr;
-    a.M = d * j.cl; a.N = j.cr; a.K = j.cr;
-    a.tiles_m = (a.M + 31) / 32; a.tiles_n = (a.N + 31) / 32;
-    a.tile_begin = t1; t1 += a.tiles_m * a.tiles_n;
-    GemmTask& b = h[N + q];
-    memset(&b, 0, sizeof(b));
-    b.A = j.A; b.lda = d * j.cr;             // conj, cl x (d cr)
-    b.B = j.X; b.ldb = d * j.cr;             // X viewed cl x (d cr), transposed
-    b.C = j.Eout; b.ldc = j.cl;
-    b.M = j.cl; b.N = j.cl; b.K = d * j.cr;
-    b.sign_div = j.z ? j.kap * j.cr : 0;
-    b.kd0 = j.d0;
-    b.kd1 = j.d1;
-    b.tiles_m = (b.M + 31) / 32; b.tiles_n = (b.N + 31) / 32;
-    b.tile_begin = t2; t2 += b.tiles_m * b.tiles_n;
+    if (left) env_tasks_left(jobs[q], h[2 * q], h[2 * q + 1]);
+    else env_tasks_right(jobs[q], h[2 * q], h[2 * q + 1]);
   }
   GemmTask* d = (GemmTask*)E->desc.p;
   HIP_TRY(hipMemcpyAsync(d, h, 2 * N * sizeof(GemmTask), hipMemcpyHostToDevice, E->stream));
-  HIP_TRY(launch_gemm(d, N, t1, 0, 3, fp32, 0, 0, E->stream));
-  HIP_TRY(launch_gemm(d + N, N, t2, 1, 3, fp32, 0, 0, E->stream));
-  HIP_TRY(hipStreamSynchronize(E->stream));
+  for (int q = 0; q < N; ++q) {
+    const GemmTask& a = h[2 * q];
+    const GemmTask& b = h[2 * q + 1];
+    if (left) {
+      HIP_TRY(launch_gemm(d + 2 * q, 1, a.tiles_m * a.tiles_n, 0, 0, 0, fp32, 0, E->stream));
+      HIP_TRY(launch_gemm(d + 2 * q + 1, 1, b.tiles_m * b.tiles_n, 2, 0, fp32, 0, 0, E->stream));
+    } else {
+      HIP_TRY(launch_gemm(d + 2 * q, 1, a.tiles_m * a.tiles_n, 0, 3, fp32, 0, 0, E->stream));
+      HIP_TRY(launch_gemm(d + 2 * q + 1, 1, b.tiles_m * b.tiles_n, 1, 3, fp32, 0, 0, E->stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(E->stream));  // staging reuse
   return 0;
 }
 
@@ -1318,19 +1344,18 @@ int compute_left_envs_diag(hipmps_state_s* s, const EnvLayout& L, const std::vec
   hipmps_engine E = s->e;
   char* base = (char*)E->env.p;
   HIP_TRY(launch_fill_identity1((double2*)(base + L.offL[0]), E->stream));
+  std::vector<EnvJob> jobs(s->n);
   for (int k = 0; k < s->n; ++k) {
-    std::vector<EnvJob> jobs(1);
-    jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offL[k]),
+    jobs[k] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offL[k]),
                (double2*)(base + L.offX), (double2*)(base + L.offL[k + 1]), 0};
-    jobs[0].kap = s->kap[k];
+    jobs[k].kap = s->kap[k];
     if (!diag.empty() && !(diag[2 * k] == 1.0 && diag[2 * k + 1] == 1.0)) {
-      jobs[0].z = 1;
-      jobs[0].d0 = diag[2 * k];
-      jobs[0].d1 = diag[2 * k + 1];
+      jobs[k].z = 1;
+      jobs[k].d0 = diag[2 * k];
+      jobs[k].d1 = diag[2 * k + 1];
     }
-    RC_TRY(env_left(E, jobs, s->fp32));
   }
-  return 0;
+  return env_chain(E, jobs, true, s->fp32);
 }
 
 int compute_left_envs(hipmps_state_s* s, const EnvLayout& L, const std::vector<int>& zmask) {
@@ -1347,14 +1372,14 @@ int compute_right_envs(hipmps_state_s* s, const EnvLayout& L) {
   hipmps_engine E = s->e;
   char* base = (char*)E->env.p;
   HIP_TRY(launch_fill_identity1((double2*)(base + L.offR[s->n]), E->stream));
+  std::vector<EnvJob> jobs;
   for (int k = s->n - 1; k >= 0; --k) {
-    std::vector<EnvJob> jobs(1);
-    jobs[0] = {s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offR[k + 1]),
-               (double2*)(base + L.offX), (double2*)(base + L.offR[k]), 0};
-    jobs[0].kap = s->kap[k];
-    RC_TRY(env_right(E, jobs, s->fp32));
+    EnvJob j{s->sites[k].p, s->chi[k], s->chi[k + 1], (const double2*)(base + L.offR[k + 1]),
+             (double2*)(base + L.offX), (double2*)(base + L.offR[k]), 0};
+    j.kap = s->kap[k];
+    jobs.push_back(j);
   }
-  return 0;
+  return env_chain(E, jobs, false, s->fp32);
 }
 
 int read_scalar(hipmps_engine E, const void* dptr, double2* out) {
