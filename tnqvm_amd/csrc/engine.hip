@@ -563,6 +563,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   }
   // double-LQ preconditioning: X = L Q~, L^H = L_Z Q_Z, Jacobi on R2 = L_Z^H
   int nlq = 0, lq_max_r = 0, lq_max_rpad = 0, lq_max_c = 0;
+  bool any_cut = false;
   for (int q = 0; q < N; ++q) {
     auto& it = items[q];
     if (!it.lq) continue;
@@ -584,11 +585,16 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     htr[nlq] = TriTask{Y, Z, it.c, it.r, it.r, it.r};               // Z = L^H
     htr[N + nlq] = TriTask{Z, Jm, it.r, it.r, it.r_pad, it.r};      // J = L_Z^H, pad rows zero
     htr[2 * N + nlq] = TriTask{Jm, Yt, it.r, it.r, it.r_pad, it.c};  // Yt = [J | 0]
-    // the singular value below which the truncation drops an index (partial norm sigma, or
-    // sqrt(sigma) with the 2-norm definition); the rank cut keeps its perturbation far below
-    {
+    // rank cut, for gates on purified sites, whose theta' has 2 chi kap rows but rank about chi
+    // (a pure site's theta' is full rank in practice, and the cut would only read it): the
+    // singular value below which the truncation drops an index (partial norm sigma, or
+    // sqrt(sigma) with the 2-norm definition), far above the cut's perturbation
+    if (it.ki > 1 || it.kj > 1) {
       const double cutoff = it.s->t.svd_cutoff;
       hcut[nlq] = it.s->t.partial_norm == HIPMPS_PN_L2 ? cutoff * cutoff : cutoff;
+      any_cut = true;
+    } else {
+      hcut[nlq] = 0.0;
     }
     ++nlq;
     lq_max_r = std::max(lq_max_r, it.r);
@@ -678,7 +684,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       HIP_TRY(launch_lower_to_upper_h(dtr + N + b, n, lq_max_r, lq_max_rpad, ls));
     }
     RC_TRY(join_lanes(lq_lanes));
-    HIP_TRY(launch_rank_cut(dtr + N, dcut, nlq, st));
+    if (any_cut) HIP_TRY(launch_rank_cut(dtr + N, dcut, nlq, st));
     for (int q = 0; q < N; ++q) {
       auto& it = items[q];
       if (!it.lq) continue;
