@@ -1421,7 +1421,9 @@ int trace_products(hipmps_engine E, const std::vector<const double2*>& A, const 
   return 0;
 }
 
-int measure_z_all(hipmps_state_s* s, double* out) {
+// <Z_i> for every site; with norm, also <psi|psi> (the closed left environment L_n against the
+// identity R_n, one more pair of the same trace products)
+int measure_z_all(hipmps_state_s* s, double* out, double* norm = nullptr) {
   hipmps_engine E = s->e;
   const int n = s->n;
   int cmax = 1;
@@ -1454,10 +1456,16 @@ int measure_z_all(hipmps_state_s* s, double* out) {
     B.push_back((const double2*)(base + L.offR[i + 1]));
     dims.push_back(s->chi[i + 1]);
   }
+  if (norm) {
+    A.push_back((const double2*)(base + L.offL[n]));
+    B.push_back((const double2*)(base + L.offR[n]));
+    dims.push_back(s->chi[n]);
+  }
   std::vector<double2> res;
   // pointer tables go right after the M buffers
   RC_TRY(trace_products(E, A, B, dims, base + L.offT + align_up(mbytes), res));
   for (int i = 0; i < n; ++i) out[i] = res[i].x;
+  if (norm) *norm = res[n].x;
   return 0;
 }
 
@@ -2408,9 +2416,9 @@ int finish_circuit(PendingCircuit* pc) {
   hipmps_result* r = pc->res;
   RC_TRY(state_flush(s));
   if (r) {
-    if (r->expval_z) RC_TRY(measure_z_all(s, r->expval_z));
+    if (r->expval_z) RC_TRY(measure_z_all(s, r->expval_z, &r->norm));
+    else RC_TRY(measure_parity(s, {}, &r->norm));
     if (r->expval_zz && !pc->zz.empty()) RC_TRY(measure_zz(s, pc->zz.data(), (int)pc->zz.size() / 2, r->expval_zz));
-    RC_TRY(measure_parity(s, {}, &r->norm));
     r->discarded_weight = s->discarded;
     r->fidelity_est = s->fid;
     r->max_bond = s->maxb;
