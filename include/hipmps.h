@@ -108,6 +108,10 @@ typedef struct {
   int norm_warnings;       /* SVDs after which a sqrt(S) factor had Frobenius norm < 1e-3: the
                               post-SVD validation of ExaTnMpsVisitor.cpp:1634-1661, which
                               prints an error there and continues (its assert is debug-only) */
+  int svds_nonconverged;   /* SVDs that reached the Jacobi sweep cap (jacobi_max_sweeps) without
+                              converging; the state's status is then HIPMPS_E_CONVERGENCE */
+  size_t device_bytes_peak; /* most site bytes held at once since creation / reset (a growing site
+                              holds its old and its new buffer during the wave) */
 } hipmps_stats_t;
 
 /* ---- engine ---------------------------------------------------------------------- */
@@ -128,6 +132,11 @@ int hipmps_state_reset(hipmps_state s); /* back to |0..0> */
 int hipmps_apply_1q(hipmps_state s, int q, const double m[8]);
 int hipmps_apply_2q(hipmps_state s, int q0, int q1, const double m[32]); /* |q0-q1| == 1 */
 int hipmps_apply_gates(hipmps_state s, int n, const hipmps_gate* gates);
+/* Gates queue into waves (2-qubit gates on disjoint sites) that run when a gate conflicts with
+ * the open wave or at a flush.  A wave whose site growth does not fit the HBM budget returns
+ * HIPMPS_E_NOMEM from the call that ran it; its 2-qubit gates are dropped and every site is left
+ * as after the last completed wave.  flush returns HIPMPS_E_CONVERGENCE (results still written)
+ * when an SVD reached the Jacobi sweep cap, HIPMPS_E_NUMERIC when NaN/Inf appeared. */
 int hipmps_flush(hipmps_state s);
 
 int hipmps_bond_dims(hipmps_state s, int* out_n_plus_1);
@@ -205,10 +214,26 @@ typedef struct {
   double latency_ms;  /* admission to completion */
 } hipmps_result;
 
+/* HBM a circuit may hold while it runs, the admission unit of hipmps_run_batch and serving: its
+ * sites at the largest bonds they can reach (min(2^k, 2^(n-k), max-bond-dim)), twice (a growing
+ * site holds its old buffer during the wave), plus its share of the wave arena.  Fails with the
+ * codes of the host checks below for a circuit they refuse. */
+int hipmps_circuit_footprint(const hipmps_circuit* c, size_t* out);
+/* Runs every circuit to completion, admitting them in order while the sum of their footprints
+ * fits hbm_budget_bytes (a circuit above the budget runs alone).  A circuit that fails alone --
+ * refused by the host checks (qubit range, adjacency, NaN/Inf: HIPMPS_E_ARG / _NONADJACENT /
+ * _NUMERIC), or unable to grow its sites within hbm_budget_bytes even after the others finished
+ * (HIPMPS_E_NOMEM) -- reports that code in results[k].status, and the other circuits' results
+ * are those they have alone.
+ * results[k].status is HIPMPS_E_CONVERGENCE when an SVD of the circuit hit the sweep cap.  The
+ * call itself returns non-zero only for an engine-level failure (bad arguments, device error). */
 int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits,
                      hipmps_result* results);
 
 /* ---- serving: continuous batching (no counterpart in the reference) ------------------ */
+/* Queues a circuit; refused at once (nothing queued) when it fails the host checks of
+ * hipmps_run_batch or when its projected footprint exceeds the HBM budget (HIPMPS_E_NOMEM).  A
+ * queued circuit that fails later fails alone, with its code in result->status. */
 int hipmps_serve_submit(hipmps_engine e, const hipmps_circuit* c, hipmps_result* result,
                         uint64_t* ticket);
 /* advance every in-flight circuit by one wave; circuits on the background lane (see
@@ -254,6 +279,9 @@ typedef struct {
   int rdm_block_elems;  /* density matrices: the per-site intermediate of the sweep is held a block of
                            (p, kappa) at a time above max(this, largest environment) complex
                            elements; default 2^24 */
+  int jacobi_max_sweeps; /* sweep cap of every Jacobi SVD (1..1000), after which the SVD counts as
+                            non-converged (HIPMPS_E_CONVERGENCE, hipmps_stats_t.svds_nonconverged);
+                            default: hipmps_engine_opts.jacobi_max_sweeps */
 } hipmps_tuning;
 int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out);
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t);

@@ -329,6 +329,32 @@ CASE(norm_check, true) {
   }
 }
 
+// Failure detection: an SVD that reaches the Jacobi sweep cap is flagged on the buffer
+// ("svd-converged" 0, with the count) instead of passing silently; the same circuit under the
+// default cap converges.  The reference has no such guard beyond its post-SVD norm check
+// (ExaTnMpsVisitor.cpp:1634-1661).
+CASE(svd_sweep_cap_flagged, true) {
+  std::mt19937_64 rng(11);
+  std::uniform_real_distribution<double> ang(-constants::pi, constants::pi);
+  auto prov = getIRProvider("quantum");
+  auto c = prov->createComposite("cap");
+  const int n = 8;
+  for (int layer = 0; layer < 8; ++layer) {
+    for (size_t q = 0; q < (size_t)n; ++q)
+      c->addInstruction(prov->createInstruction("U", {q}, {InstructionParameter(ang(rng)), InstructionParameter(ang(rng)),
+                                                           InstructionParameter(ang(rng))}));
+    for (size_t q = layer % 2; q + 1 < (size_t)n; q += 2) c->addInstruction(prov->createInstruction("CNOT", {q, q + 1}));
+  }
+  auto capped = qalloc(n);
+  mps({{"jacobi-max-sweeps", 1}})->execute(capped, c);
+  EXPECT_EQ((*capped)["svd-converged"].as<int>(), 0);
+  EXPECT_TRUE((*capped)["svd-nonconverged-count"].as<int>() > 0);
+  auto plain = qalloc(n);
+  mps()->execute(plain, c);
+  EXPECT_EQ((*plain)["svd-converged"].as<int>(), 1);
+  EXPECT_EQ((*plain)["svd-nonconverged-count"].as<int>(), 0);
+}
+
 // ITensorMPSVisitorTester.cpp:310-392 closed forms, exact
 CASE(rotation_kats, true) {
   auto rx = compile1("__qpu__ void r(qbit q, double t) { Rx(q[0], t); Measure(q[0]); }");

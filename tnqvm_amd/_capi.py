@@ -37,7 +37,7 @@ EXPORTS = [
     "hipmps_serve_poll", "hipmps_serve_pending", "hipmps_engine_profile", "hipmps_engine_set_timing", "hipmps_engine_set_lq_precondition",
     "hipmps_expval_diag", "hipmps_state_clone", "hipmps_apply_kraus", "hipmps_kraus_dims",
     "hipmps_reduced_density_matrix", "hipmps_engine_get_tuning", "hipmps_engine_set_tuning",
-    "hipmps_queue_gates", "hipmps_queue_kraus", "hipmps_run_waves",
+    "hipmps_queue_gates", "hipmps_queue_kraus", "hipmps_run_waves", "hipmps_circuit_footprint",
 ]
 
 
@@ -66,7 +66,7 @@ class Gate(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("jacobi_lanes", ctypes.c_int), ("lane_min_pairs", ctypes.c_int), ("gram32_sweeps", ctypes.c_int),
                 ("lq_min_rank", ctypes.c_int), ("big_lane_min_chi", ctypes.c_int),
-                ("rdm_block_elems", ctypes.c_int)]
+                ("rdm_block_elems", ctypes.c_int), ("jacobi_max_sweeps", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -74,7 +74,8 @@ class Stats(ctypes.Structure):
                 ("fidelity_est", ctypes.c_double), ("max_bond", ctypes.c_int), ("n_2q", ctypes.c_int),
                 ("n_1q", ctypes.c_int), ("jacobi_sweeps_max", ctypes.c_int),
                 ("jacobi_sweeps_total", ctypes.c_longlong), ("device_bytes", ctypes.c_size_t),
-                ("norm_warnings", ctypes.c_int)]
+                ("norm_warnings", ctypes.c_int), ("svds_nonconverged", ctypes.c_int),
+                ("device_bytes_peak", ctypes.c_size_t)]
 
 
 class Circuit(ctypes.Structure):

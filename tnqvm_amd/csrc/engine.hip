@@ -243,6 +243,9 @@ struct hipmps_state_s {
   size_t bytes = 0;
   int status = 0;
   int norm_warnings = 0;  // SVDs whose factor norm fell below 1e-3 (post-SVD validation)
+  int nonconverged = 0;   // SVDs that reached the Jacobi sweep cap without converging
+  size_t peak = 0;        // most site bytes held at once (old and new buffers during a growth)
+  size_t wave_new = 0;    // scratch of grow_sites
   int kraus_ops = 0;      // noise channels applied
   std::vector<void*> deferred;
 };
@@ -284,6 +287,38 @@ size_t projected_bytes(int n, int maxb, int fp32) {
   return tot;
 }
 
+// largest bond a circuit can reach: min(max-bond-dim, 2^floor(n/2))
+int projected_max_bond(const hipmps_circuit& c) {
+  const int half = c.n_qubits / 2;
+  const long long full = half >= 30 ? (1LL << 30) : (1LL << half);
+  const long long cap = c.trunc.max_bond_dim > 0 ? c.trunc.max_bond_dim : (1LL << 30);
+  return (int)std::min(full, cap);
+}
+
+// HBM a circuit may hold while it runs, for admission: its sites at the largest bonds they can
+// reach, twice (a growing site holds its old buffer until theta has read it), plus its share of
+// the wave arena (theta', the Jacobi matrix and the LQ factors of up to n/2 gates of bond
+// chi: about 5 (2 chi)^2 complex128 each)
+size_t circuit_footprint(const hipmps_circuit& c) {
+  const int maxb = projected_max_bond(c);
+  const size_t sites = projected_bytes(c.n_qubits, maxb, c.trunc.precision == HIPMPS_MIXED);
+  const size_t gates = (size_t)std::max(1, c.n_qubits / 2);
+  return 2 * sites + gates * 5 * (size_t)(2 * maxb) * (size_t)(2 * maxb) * 16;
+}
+
+// capacity a pure site grows to: doubled, but never past the largest the site can ever hold
+// (bonds bounded by min(2^k, 2^(n-k), max-bond-dim)), so that a state never holds more than
+// its projected footprint
+size_t grown_capacity(const hipmps_state_s* s, int k, size_t need, size_t cap) {
+  size_t ncap = std::max(need, cap * 2);
+  if (s->kap[k] == 1) {
+    const int maxb = s->t.max_bond_dim > 0 ? s->t.max_bond_dim : INT_MAX - 1;
+    const size_t bound = bond_bound(s->n, k, maxb) * 2 * bond_bound(s->n, k + 1, maxb);
+    ncap = std::max(need, std::min(ncap, bound));
+  }
+  return ncap;
+}
+
 size_t site_elems(const hipmps_state_s* s, int k) { return (size_t)s->chi[k] * 2 * s->kap[k] * s->chi[k + 1]; }
 
 bool is_purified(const hipmps_state_s* s) {
@@ -322,12 +357,60 @@ int ensure_site(hipmps_state_s* s, int k, size_t elems, bool keep_old_for_read, 
   S.cap = ncap;
   e->used += nbytes;
   s->bytes += nbytes;
+  s->peak = std::max(s->peak, s->bytes);
   return 0;
 }
 
 void release_deferred(hipmps_state_s* s) {
   for (void* p : s->deferred) (void)hipFree(p);
   s->deferred.clear();
+}
+
+// Site growth of a whole wave, in two phases: every new buffer is reserved (budget) and
+// allocated first; only when all of them exist are they committed, the old buffers parked in
+// `deferred` for theta to read.  On failure every new buffer is freed again and no state of the
+// wave is touched, so the wave can be retried (or its states run apart) with their sites intact.
+struct SiteGrowth {
+  hipmps_state_s* s;
+  int k;
+  size_t ncap;  // elements
+  void* p;
+};
+
+int grow_sites(hipmps_engine E, std::vector<SiteGrowth>& g) {
+  size_t add = 0;
+  for (auto& x : g) add += x.ncap * x.s->es;
+  if (E->used + add > E->budget)
+    return fail(HIPMPS_E_NOMEM, "HBM budget exceeded growing " + std::to_string(g.size()) + " site(s) by " +
+                                    std::to_string(add) + " bytes");
+  for (size_t a = 0; a < g.size(); ++a) {
+    if (hipMalloc(&g[a].p, g[a].ncap * g[a].s->es) != hipSuccess) {
+      (void)hipGetLastError();
+      for (size_t b = 0; b < a; ++b) (void)hipFree(g[b].p);
+      for (auto& x : g) x.p = nullptr;
+      return fail(HIPMPS_E_NOMEM, "device allocation for site " + std::to_string(g[a].k) + " failed");
+    }
+  }
+  for (auto& x : g) x.s->wave_new += x.ncap * x.s->es;
+  for (auto& x : g) {
+    if (x.s->wave_new) {  // held during the wave: the old buffers and the new ones
+      x.s->peak = std::max(x.s->peak, x.s->bytes + x.s->wave_new);
+      x.s->wave_new = 0;
+    }
+  }
+  for (auto& x : g) {
+    Site& S = x.s->sites[x.k];
+    if (S.p) {
+      x.s->deferred.push_back(S.p);
+      E->used -= S.cap * x.s->es;
+      x.s->bytes -= S.cap * x.s->es;
+    }
+    S.p = x.p;
+    S.cap = x.ncap;
+    E->used += x.ncap * x.s->es;
+    x.s->bytes += x.ncap * x.s->es;
+  }
+  return 0;
 }
 
 int upload_site_host(hipmps_state_s* s, int k, const cd* data, size_t n) {
@@ -371,6 +454,8 @@ int state_init_product(hipmps_state_s* s) {
   s->sweeps_total = 0;
   s->status = 0;
   s->norm_warnings = 0;
+  s->nonconverged = 0;
+  s->peak = s->bytes;
   s->kraus_ops = 0;
   return 0;
 }
@@ -422,11 +507,6 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   const int N = (int)items.size();
   for (auto& it : items)
     if (it.r > 4096) return fail(HIPMPS_E_UNSUPPORTED, "bond dimension above 2048 is not supported");
-  // site capacity (new sites are written after theta has read the old ones)
-  for (auto& it : items) {
-    RC_TRY(ensure_site(it.s, it.i, (size_t)it.cl * 2 * it.kub, true, &it.old_i));
-    RC_TRY(ensure_site(it.s, it.i + 1, (size_t)it.kub * 2 * it.cr, true, &it.old_j));
-  }
   // arena
   size_t off = 0;
   for (auto& it : items) {
@@ -472,6 +552,20 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   // host staging: descriptors | counts (2N ints) | active Jacobi list / results
   const size_t stage_bytes = desc_bytes + align_up(2 * N * sizeof(int)) + N * sizeof(JacTask) + WS_N * N * sizeof(double) + 4096;
   RC_TRY(E->hdesc.ensure(stage_bytes));
+  // site capacity, after every engine buffer exists (new sites are written after theta has
+  // read the old ones); all-or-nothing across the wave
+  {
+    std::vector<SiteGrowth> grow;
+    for (auto& it : items) {
+      const size_t need[2] = {(size_t)it.cl * 2 * it.kub, (size_t)it.kub * 2 * it.cr};
+      for (int h = 0; h < 2; ++h) {
+        Site& S = it.s->sites[it.i + h];
+        (h ? it.old_j : it.old_i) = S.p;
+        if (need[h] > S.cap) grow.push_back({it.s, it.i + h, grown_capacity(it.s, it.i + h, need[h], S.cap), nullptr});
+      }
+    }
+    if (!grow.empty()) RC_TRY(grow_sites(E, grow));
+  }
   char* hd = (char*)E->hdesc.p;
   ThetaTask* hth = (ThetaTask*)hd;
   SplitTask* hsp = (SplitTask*)(hd + align_up(N * sizeof(ThetaTask)));
@@ -701,8 +795,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   std::vector<int> active(N);
   for (int q = 0; q < N; ++q) active[q] = q;
   std::vector<int> sweeps_used(N, E->max_sweeps);
+  std::vector<char> converged(N, 0);
   int* hcounts = (int*)((char*)E->hdesc.p + desc_bytes);
-  bool failed_conv = false;
   for (int sweep = 0; sweep < E->max_sweeps && !active.empty(); ++sweep) {
     // active task list with pair prefix; with several lanes, the items are dealt to lists
     // of about equal work (largest first), each a prefix-indexed launch on its own stream
@@ -780,12 +874,15 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       const double c = items[q].lq ? items[q].r : items[q].c;  // columns the Jacobi works on
       E->prof.jacobi_flops += 8.0 * JP * JP * c * (double)(vis + rot);
       const bool g32 = sweep < E->gram32_sweeps && items[q].fp32 && hjt[q].r >= 256;
-      if (rot == 0 && !g32) sweeps_used[q] = sweep + 1;
-      else next.push_back(q);
+      if (rot == 0 && !g32) {
+        sweeps_used[q] = sweep + 1;
+        converged[q] = 1;
+      } else {
+        next.push_back(q);
+      }
     }
     active.swap(next);
   }
-  if (!active.empty()) failed_conv = true;
   if (nlq > 0) {  // rows of the Jacobi result back to the column space of X
     RC_TRY(fork_lanes(lq_lanes));
     for (int l = 0; l < lq_lanes; ++l) {
@@ -871,7 +968,10 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     s->n2q++;
     s->sweeps_max = std::max(s->sweeps_max, sweeps_used[q]);
     s->sweeps_total += sweeps_used[q];
-    if (failed_conv && sweeps_used[q] >= E->max_sweeps) s->status = HIPMPS_E_CONVERGENCE;
+    if (!converged[q]) {
+      if (s->status == 0) s->status = HIPMPS_E_CONVERGENCE;  // NaN / Inf is the graver report
+      s->nonconverged++;
+    }
   }
   for (auto& it : items) release_deferred(it.s);
   return 0;
@@ -894,7 +994,9 @@ int state_run_wave(hipmps_state_s* s) {
       it.G = q.G;
       items.push_back(it);
     }
-    int rc = run_wave(s->e, items);
+    // growth is all or nothing: when the wave cannot run (HIPMPS_E_NOMEM), its 2-qubit gates
+    // are dropped with the error and the sites stay those of the last completed wave
+    const int rc = run_wave(s->e, items);
     s->wave.clear();
     std::fill(s->busy.begin(), s->busy.end(), 0);
     if (rc != 0) return rc;
@@ -1128,20 +1230,35 @@ int run_kraus_waves(hipmps_engine E, const std::vector<hipmps_state_s*>& states)
       if (hcount[2 * i] != 0) next.push_back(i);
     active.swap(next);
   }
-  for (int i : active) ch[i].s->status = HIPMPS_E_CONVERGENCE;
+  for (int i : active) {
+    if (ch[i].s->status == 0) ch[i].s->status = HIPMPS_E_CONVERGENCE;
+    ch[i].s->nonconverged++;
+  }
   HIP_TRY(launch_rownorm(dsp, C, max_rows, st));
   HIP_TRY(launch_select(dsp, C, st));
   HIP_TRY(hipMemcpyAsync(hcount + 2 * C, d_K, (size_t)C * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(hw, d_w, (size_t)C * WS_N * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  // new Kraus extents: every site that must grow is reserved before any is written (all or
+  // nothing, as in run_wave), so a failure leaves every state and its queued channels as they were
+  {
+    std::vector<SiteGrowth> grow;
+    for (int i = 0; i < C; ++i) {
+      const Ch& h = ch[i];
+      const int Kn = hcount[2 * C + i];
+      if (Kn < 1 || Kn > h.jr) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
+      const size_t need = (size_t)h.cl * 2 * Kn * h.cr;
+      const Site& S = h.s->sites[h.q];
+      if (need > S.cap) grow.push_back({h.s, h.q, std::max(need, S.cap * 2), nullptr});
+    }
+    if (!grow.empty()) RC_TRY(grow_sites(E, grow));
+  }
   for (int i = 0; i < C; ++i) {
     const Ch& h = ch[i];
     hipmps_state_s* s = h.s;
     const int Kn = hcount[2 * C + i];
-    if (Kn < 1 || Kn > h.jr) return fail(HIPMPS_E_NUMERIC, "Kraus compression returned no rows");
     const double* w = hw + (size_t)i * WS_N;
     if (w[5] != 0.0 || !std::isfinite(w[0] + w[1])) s->status = HIPMPS_E_NUMERIC;
-    RC_TRY(ensure_site(s, h.q, (size_t)h.cl * 2 * Kn * h.cr, false, nullptr));
     const int* perm = (const int*)(ws + h.off_v + 40 * (size_t)h.r_pad);
     if (h.adj)
       HIP_TRY(launch_kraus_gather_adj((double2*)(ws + h.off_x), h.ld, h.rows, (const double*)(ws + h.off_v), perm,
@@ -1156,6 +1273,7 @@ int run_kraus_waves(hipmps_engine E, const std::vector<hipmps_state_s*>& states)
   for (auto* st_ : states) {
     st_->cwave.clear();
     std::fill(st_->cbusy.begin(), st_->cbusy.end(), 0);
+    release_deferred(st_);
   }
   return 0;
 }
@@ -2026,13 +2144,15 @@ int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out) {
   out->lq_min_rank = e->lq_min_rank;
   out->big_lane_min_chi = e->big_min_chi;
   out->rdm_block_elems = e->rdm_block_elems;
+  out->jacobi_max_sweeps = e->max_sweeps;
   return 0;
 }
 
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   if (!e || !t) return fail(HIPMPS_E_ARG, "null argument");
   if (t->jacobi_lanes < 1 || t->jacobi_lanes > hipmps_engine_s::MAX_LANES || t->lane_min_pairs < 0 ||
-      t->gram32_sweeps < 0 || t->lq_min_rank < 0 || t->big_lane_min_chi < 0 || t->rdm_block_elems < 1)
+      t->gram32_sweeps < 0 || t->lq_min_rank < 0 || t->big_lane_min_chi < 0 || t->rdm_block_elems < 1 ||
+      t->jacobi_max_sweeps < 1 || t->jacobi_max_sweeps > 1000)
     return fail(HIPMPS_E_ARG, "tuning values out of range");
   std::lock_guard<std::mutex> lock(e->mu);
   int made = 1;  // lanes whose stream exists
@@ -2043,6 +2163,7 @@ int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   e->lq_min_rank = t->lq_min_rank;
   e->big_min_chi = t->big_lane_min_chi;
   e->rdm_block_elems = t->rdm_block_elems;
+  e->max_sweeps = t->jacobi_max_sweeps;
   return 0;
 }
 
@@ -2132,7 +2253,10 @@ int hipmps_apply_gates(hipmps_state s, int n, const hipmps_gate* gates) {
 int hipmps_flush(hipmps_state s) {
   if (!s) return fail(HIPMPS_E_ARG, "null state");
   RC_TRY(state_flush(s));
-  return s->status;
+  if (s->status == HIPMPS_E_CONVERGENCE)
+    return fail(s->status, std::to_string(s->nonconverged) + " SVD(s) reached the Jacobi sweep cap unconverged");
+  if (s->status != 0) return fail(s->status, "a tensor of the state held NaN or Inf");
+  return 0;
 }
 
 int hipmps_apply_kraus(hipmps_state s, int q, int n_ops, const double* mats) {
@@ -2248,6 +2372,8 @@ int hipmps_stats(hipmps_state s, hipmps_stats_t* out) {
   out->jacobi_sweeps_total = s->sweeps_total;
   out->device_bytes = s->bytes;
   out->norm_warnings = s->norm_warnings;
+  out->svds_nonconverged = s->nonconverged;
+  out->device_bytes_peak = s->peak;
   return s->status;
 }
 
@@ -2321,6 +2447,7 @@ int hipmps_state_clone(hipmps_state src, hipmps_state* out) {
   dst->sweeps_total = src->sweeps_total;
   dst->status = src->status;
   dst->norm_warnings = src->norm_warnings;
+  dst->nonconverged = src->nonconverged;
   *out = dst;
   return 0;
 }
@@ -2429,37 +2556,118 @@ int finish_circuit(PendingCircuit* pc) {
   return 0;
 }
 
+// A circuit that fails on its own (a site it cannot grow within the budget, a gate the host
+// refuses) ends with that status; the circuits beside it go on.
+void fail_circuit(PendingCircuit* pc, int rc) {
+  if (pc->res) {
+    pc->res->status = rc;
+    pc->res->latency_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pc->t0).count();
+  }
+  pc->finished = true;
+}
+
+std::vector<WaveItem> wave_items(hipmps_state_s* s) {
+  std::vector<WaveItem> v;
+  for (auto& q : s->wave) {
+    WaveItem it{};
+    it.s = s;
+    it.i = q.i;
+    it.G = q.G;
+    v.push_back(it);
+  }
+  return v;
+}
+
 // Feed each running circuit up to its next wave conflict, then run all waves together.
+// Memory: a wave grows its sites all or nothing (run_wave), so when the joint wave does not fit,
+// the circuits' waves run one by one; a circuit whose own wave does not fit waits (its wave stays
+// queued) while others progress and free memory, and fails with HIPMPS_E_NOMEM only when no
+// circuit could progress, the largest first.
 int advance_all(hipmps_engine E, std::vector<PendingCircuit*>& run) {
   std::vector<WaveItem> items;
+  std::vector<PendingCircuit*> live;
   for (auto* pc : run) {
     if (pc->finished) continue;
     hipmps_state_s* s = pc->st;
+    int rc = 0;
     while (pc->cursor < pc->gates.size()) {
       const hipmps_gate& g = pc->gates[pc->cursor];
       if (g.nq == 2) {
         const int i = std::min(g.q0, g.q1);
-        if (s->busy[i] || s->busy[i + 1]) break;  // next wave
+        if (i >= 0 && i + 1 < s->n && (s->busy[i] || s->busy[i + 1])) break;  // next wave
       }
-      RC_TRY(apply_gate_list(s, 1, &g));
+      rc = apply_gate_list(s, 1, &g);
+      if (rc != 0) break;
       pc->cursor++;
     }
-    for (auto& q : s->wave) {
-      WaveItem it{};
-      it.s = s;
-      it.i = q.i;
-      it.G = q.G;
-      items.push_back(it);
+    if (rc != 0) {
+      fail_circuit(pc, rc);
+      continue;
     }
+    live.push_back(pc);
+    for (auto& it : wave_items(s)) items.push_back(it);
   }
-  RC_TRY(run_wave(E, items));
-  for (auto* pc : run) {
-    if (pc->finished) continue;
+  std::vector<char> ran(live.size(), 1);
+  int rc = run_wave(E, items);
+  if (rc == HIPMPS_E_NOMEM) {
+    bool progress = false;
+    for (size_t k = 0; k < live.size(); ++k) {
+      std::vector<WaveItem> one = wave_items(live[k]->st);
+      const int r = run_wave(E, one);
+      if (r == HIPMPS_E_NOMEM) ran[k] = 0;
+      else if (r != 0) return r;
+      else progress = true;
+    }
+    if (!progress) {  // nothing can move: the largest stalled circuit gives way
+      size_t worst = 0;
+      for (size_t k = 1; k < live.size(); ++k)
+        if (live[k]->st->bytes > live[worst]->st->bytes) worst = k;
+      fail_circuit(live[worst], HIPMPS_E_NOMEM);
+    }
+  } else if (rc != 0) {
+    return rc;
+  }
+  for (size_t k = 0; k < live.size(); ++k) {
+    PendingCircuit* pc = live[k];
+    if (pc->finished || !ran[k]) continue;
     hipmps_state_s* s = pc->st;
     s->wave.clear();
     std::fill(s->busy.begin(), s->busy.end(), 0);
-    if (pc->cursor >= pc->gates.size()) RC_TRY(finish_circuit(pc));
+    if (pc->cursor >= pc->gates.size()) {
+      const int r = finish_circuit(pc);
+      if (r == HIPMPS_E_NOMEM) fail_circuit(pc, r);  // its measurement workspace does not fit
+      else if (r != 0) return r;
+    }
   }
+  return 0;
+}
+
+// Host-side checks of a submitted circuit, so that a bad request is refused at the door and
+// cannot stall the circuits it would run beside.
+int validate_circuit(const hipmps_circuit& c) {
+  if (c.n_qubits < 1) return fail(HIPMPS_E_ARG, "a circuit needs at least one qubit");
+  if (c.n_gates < 0 || (c.n_gates > 0 && !c.gates)) return fail(HIPMPS_E_ARG, "bad gate list");
+  if (c.n_zz < 0 || (c.n_zz > 0 && !c.zz_pairs)) return fail(HIPMPS_E_ARG, "bad <ZZ> pair list");
+  if (c.trunc.precision != HIPMPS_FP64 && c.trunc.precision != HIPMPS_MIXED)
+    return fail(HIPMPS_E_ARG, "unknown precision");
+  const int n = c.n_qubits;
+  for (int k = 0; k < c.n_gates; ++k) {
+    const hipmps_gate& g = c.gates[k];
+    const std::string at = "gate " + std::to_string(k) + ": ";
+    if (g.nq == 1) {
+      if (g.q0 < 0 || g.q0 >= n) return fail(HIPMPS_E_ARG, at + "qubit index out of range");
+    } else if (g.nq == 2) {
+      if (g.q0 < 0 || g.q0 >= n || g.q1 < 0 || g.q1 >= n || g.q0 == g.q1)
+        return fail(HIPMPS_E_ARG, at + "qubit index out of range");
+      if (std::abs(g.q0 - g.q1) != 1) return fail(HIPMPS_E_NONADJACENT, at + "2-qubit gate on non-neighbouring qubits");
+    } else {
+      return fail(HIPMPS_E_ARG, at + "a gate acts on 1 or 2 qubits");
+    }
+    if (!gate_finite(g)) return fail(HIPMPS_E_NUMERIC, at + "matrix holds NaN or Inf");
+  }
+  for (int k = 0; k < 2 * c.n_zz; ++k)
+    if (c.zz_pairs[k] < 0 || c.zz_pairs[k] >= n) return fail(HIPMPS_E_ARG, "<ZZ> pair qubit out of range");
   return 0;
 }
 
@@ -2537,42 +2745,73 @@ int ensure_big_lane(hipmps_engine E) {
   return 0;
 }
 
-// largest bond a circuit can reach: min(max-bond-dim, 2^floor(n/2))
-int projected_max_bond(const hipmps_circuit& c) {
-  const int half = c.n_qubits / 2;
-  const long long full = half >= 30 ? (1LL << 30) : (1LL << half);
-  const long long cap = c.trunc.max_bond_dim > 0 ? c.trunc.max_bond_dim : (1LL << 30);
-  return (int)std::min(full, cap);
-}
-
 }  // namespace
 
 extern "C" {
+
+int hipmps_circuit_footprint(const hipmps_circuit* c, size_t* out) {
+  if (!c || !out) return fail(HIPMPS_E_ARG, "null argument");
+  RC_TRY(validate_circuit(*c));
+  *out = circuit_footprint(*c);
+  return 0;
+}
 
 int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits, hipmps_result* results) {
   if (!e || n < 0 || (n > 0 && (!circuits || !results))) return fail(HIPMPS_E_ARG, "bad arguments");
   std::lock_guard<std::mutex> lock(e->mu);
   RC_TRY(set_device(e));
-  std::vector<PendingCircuit*> run;
-  int rc = 0;
+  std::vector<PendingCircuit*> all, queue, running;
   auto t0 = std::chrono::steady_clock::now();
-  for (int k = 0; k < n && rc == 0; ++k) {
+  for (int k = 0; k < n; ++k) {
     auto* pc = new PendingCircuit();
+    pc->res = &results[k];
+    pc->t0 = t0;
+    all.push_back(pc);
+    const int v = validate_circuit(circuits[k]);
+    if (v != 0) {  // refused alone; the others run
+      fail_circuit(pc, v);
+      continue;
+    }
     pc->c = circuits[k];
     pc->gates.assign(circuits[k].gates, circuits[k].gates + circuits[k].n_gates);
     if (circuits[k].n_zz > 0) pc->zz.assign(circuits[k].zz_pairs, circuits[k].zz_pairs + 2 * circuits[k].n_zz);
-    pc->res = &results[k];
-    pc->t0 = t0;
-    run.push_back(pc);
-    rc = start_circuit(e, pc);
+    pc->footprint = circuit_footprint(circuits[k]);
+    queue.push_back(pc);
   }
-  while (rc == 0) {
-    bool all_done = true;
-    for (auto* pc : run) all_done &= pc->finished;
-    if (all_done) break;
-    rc = advance_all(e, run);
+  // admission by projected footprint, in submission order (a circuit larger than the budget
+  // runs when it is alone); finished circuits free their sites at once
+  int rc = 0;
+  size_t next = 0, inflight = 0;
+  while (rc == 0 && (next < queue.size() || !running.empty())) {
+    while (next < queue.size() && (running.empty() || inflight + queue[next]->footprint <= e->budget)) {
+      PendingCircuit* pc = queue[next++];
+      const int r = start_circuit(e, pc);
+      if (r == HIPMPS_E_NOMEM) {
+        fail_circuit(pc, r);
+        continue;
+      }
+      if (r != 0) {
+        rc = r;
+        break;
+      }
+      running.push_back(pc);
+      inflight += pc->footprint;
+    }
+    if (rc != 0) break;
+    rc = advance_all(e, running);
+    std::vector<PendingCircuit*> keep;
+    for (auto* pc : running) {
+      if (pc->finished) {
+        hipmps_state_destroy(pc->st);
+        pc->st = nullptr;
+        inflight -= pc->footprint;
+      } else {
+        keep.push_back(pc);
+      }
+    }
+    running.swap(keep);
   }
-  for (auto* pc : run) {
+  for (auto* pc : all) {
     if (pc->st) hipmps_state_destroy(pc->st);
     delete pc;
   }
@@ -2582,16 +2821,13 @@ int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits, hip
 int hipmps_serve_submit(hipmps_engine e, const hipmps_circuit* c, hipmps_result* result, uint64_t* ticket) {
   if (!e || !c || !result || !ticket) return fail(HIPMPS_E_ARG, "bad arguments");
   std::lock_guard<std::mutex> lock(e->mu);
-  for (int k = 0; k < c->n_gates; ++k)
-    if (!gate_finite(c->gates[k])) return fail(HIPMPS_E_NUMERIC, "gate " + std::to_string(k) + " holds NaN or Inf");
+  RC_TRY(validate_circuit(*c));
   auto* pc = new PendingCircuit();
   pc->c = *c;
   pc->gates.assign(c->gates, c->gates + c->n_gates);
   if (c->n_zz > 0) pc->zz.assign(c->zz_pairs, c->zz_pairs + 2 * c->n_zz);
   pc->res = result;
-  int maxb = c->trunc.max_bond_dim > 0 ? c->trunc.max_bond_dim : 4096;
-  pc->footprint = projected_bytes(c->n_qubits, maxb, c->trunc.precision == HIPMPS_MIXED) +
-                  (size_t)2 * (2 * maxb) * (2 * maxb) * 16;
+  pc->footprint = circuit_footprint(*c);
   if (pc->footprint > e->budget) {
     delete pc;
     return fail(HIPMPS_E_NOMEM, "circuit's projected footprint exceeds the HBM budget");
@@ -2625,7 +2861,9 @@ int hipmps_serve_step(hipmps_engine e, int* n_done) {
       }
       e->big_cv.notify_all();
     } else {
-      RC_TRY(start_circuit(e, pc));
+      const int r = start_circuit(e, pc);
+      if (r == HIPMPS_E_NOMEM) fail_circuit(pc, r);
+      else RC_TRY(r);
     }
     e->running.push_back(pc);
     inflight += pc->footprint;

@@ -74,7 +74,18 @@ void HipMpsVisitor::initialize(std::shared_ptr<AcceleratorBuffer> in_buffer, int
     hipmps_state_destroy(m_state);
     m_state = nullptr;
   }
-  check(hipmps_state_create(engine(device), m_n, &t, &m_state), "state creation");
+  check(hipmps_state_create(run_engine(device), m_n, &t, &m_state), "state creation");
+}
+
+// The engine is shared by every run on the device; its Jacobi sweep cap is set from this run's
+// "jacobi-max-sweeps" option (default 40).
+hipmps_engine HipMpsVisitor::run_engine(int device) const {
+  hipmps_engine e = engine(device);
+  hipmps_tuning tu{};
+  check(hipmps_engine_get_tuning(e, &tu), "tuning");
+  tu.jacobi_max_sweeps = options.keyExists<int>("jacobi-max-sweeps") ? options.get<int>("jacobi-max-sweeps") : 40;
+  check(hipmps_engine_set_tuning(e, &tu), "jacobi-max-sweeps");
+  return e;
 }
 
 void HipMpsVisitor::apply(xacc::Instruction& g) {
@@ -181,6 +192,12 @@ void HipMpsVisitor::finalize() {
   buffer->addExtraInfo("norm", norm);
   hipmps_stats_t st{};
   hipmps_stats(m_state, &st);
+  // an SVD that hit the Jacobi sweep cap: results are still written, flagged on the buffer
+  buffer->addExtraInfo("svd-converged", st.svds_nonconverged == 0 ? 1 : 0);
+  buffer->addExtraInfo("svd-nonconverged-count", st.svds_nonconverged);
+  if (st.svds_nonconverged > 0)
+    std::cerr << "[WARNING] " << st.svds_nonconverged
+              << " SVD(s) reached the Jacobi sweep cap without converging; results may be inaccurate\n";
   buffer->addExtraInfo("discarded-weight", st.discarded_weight);
   buffer->addExtraInfo("truncation-fidelity", st.fidelity_est);
   buffer->addExtraInfo("max-bond-dim-reached", st.max_bond);

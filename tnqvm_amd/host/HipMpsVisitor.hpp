@@ -83,6 +83,8 @@ class HipMpsVisitor : public TNQVMVisitor {
   std::vector<std::string> sample_sequential(int shots);
   // truncation options from the shared option keys, with the given default cutoff
   hipmps_trunc_opts trunc_options(double default_cutoff) const;
+  // the device's shared engine with this run's Jacobi sweep cap ("jacobi-max-sweeps")
+  hipmps_engine run_engine(int device) const;
 
   hipmps_state m_state = nullptr;
   int m_n = 0;

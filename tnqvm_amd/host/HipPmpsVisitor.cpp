@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <iostream>
 
 #include "RandomEngine.hpp"
 
@@ -29,7 +30,7 @@ void HipPmpsVisitor::initialize(std::shared_ptr<AcceleratorBuffer> in_buffer, in
     hipmps_state_destroy(m_state);
     m_state = nullptr;
   }
-  check(hipmps_state_create(engine(device), m_n, &t, &m_state), "state creation");
+  check(hipmps_state_create(run_engine(device), m_n, &t, &m_state), "state creation");
 }
 
 // applySingleQubitGate / applyTwoQubitGate tails (ExaTnPmpsVisitor.cpp:798-808, :836-846):
@@ -70,6 +71,13 @@ void HipPmpsVisitor::finalize() {
   double tr = 0.0;
   check(hipmps_norm(m_state, &tr), "trace");
   buffer->addExtraInfo("norm", tr);
+  hipmps_stats_t st{};
+  hipmps_stats(m_state, &st);
+  buffer->addExtraInfo("svd-converged", st.svds_nonconverged == 0 ? 1 : 0);
+  buffer->addExtraInfo("svd-nonconverged-count", st.svds_nonconverged);
+  if (st.svds_nonconverged > 0)
+    std::cerr << "[WARNING] " << st.svds_nonconverged
+              << " SVD(s) reached the Jacobi sweep cap without converging; results may be inaccurate\n";
   std::vector<int> kd(m_n);
   check(hipmps_kraus_dims(m_state, kd.data()), "Kraus dims");
   buffer->addExtraInfo("kraus-dims", kd);

@@ -135,6 +135,14 @@ class Engine:
             c.zz_pairs = zarr.ctypes.data_as(_ip)
         return c, (garr, zarr)
 
+    def footprint(self, n, gates, trunc=None) -> int:
+        """Admission footprint of a circuit in bytes (hipmps_circuit_footprint)."""
+        c, hold = self._circuit(n, gates, trunc or trunc_opts(), None)
+        out = ctypes.c_size_t()
+        check(lib().hipmps_circuit_footprint(ctypes.byref(c), ctypes.byref(out)))
+        del hold
+        return out.value
+
     def run_batch(self, jobs):
         """jobs: list of dicts {n, gates, trunc (TruncOpts), zz (list of pairs)}.
         Returns list of dicts with expval_z, expval_zz, norm, discarded_weight, ..."""
