@@ -147,6 +147,19 @@ int hipmps_expval_z_all(hipmps_state s, double* out_n);
 int hipmps_expval_zz(hipmps_state s, const int* pairs, int k, double* out);
 int hipmps_amplitudes(hipmps_state s, const int8_t* bits, int k, double* out_re_im);
 int hipmps_wavefunction_slice(hipmps_state s, const int* bits, double* out_re_im, size_t cap);
+/* hipmps_amplitudes for callers that cannot allocate memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  It allocates
+ * nothing: every intermediate lives in a static device array of HIPMPS_NOALLOC_SCRATCH_BYTES that
+ * each GPU holds from library load to process end, and results equal hipmps_amplitudes' bit for
+ * bit.  Limits and rules:
+ *   - it needs n x 72 + k x n + 2 x k x max_bond x 16 bytes (each part rounded up to 256); a
+ *     larger request fails with HIPMPS_E_NOMEM (16 MiB holds 2000 strings at bond 256);
+ *   - the state must have no queued gates (hipmps_flush it first; a flush may grow site tensors),
+ *     else HIPMPS_E_ARG; purified states are refused as in hipmps_amplitudes;
+ *   - all calls on one GPU share its array: calls that overlap on one GPU (two threads, two
+ *     engines on the same device) race and must be serialised by the caller. */
+#define HIPMPS_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+int hipmps_amplitudes_noalloc(hipmps_state s, const int8_t* bits, int k, double* out_re_im);
 /* <psi| (x)_k diag(d[2k], d[2k+1]) |psi>: Z strings, projectors (sequential sampling,
  * getMeasureSample :2211-2364), occupation numbers */
 int hipmps_expval_diag(hipmps_state s, const double* diag_2n, double* out);

@@ -43,6 +43,13 @@
 using cd = std::complex<double>;
 using namespace hipmps;
 
+// Static device scratch of hipmps_amplitudes_noalloc: the chain vectors, the bit strings and the
+// chain descriptors of one call.  Every GPU the library runs on holds its own copy from module
+// load to process end.  Host code never passes this name to a kernel or a copy: it is a host-side
+// stand-in, and the device address of each GPU's copy comes from hipGetSymbolAddress, taken once
+// per device when an engine is created on it (engine_scratch_setup).
+static __device__ __align__(256) char scratch[HIPMPS_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string t_err;
@@ -199,6 +206,8 @@ struct hipmps_engine_s {
   // density-matrix sweep: the per-site intermediate is held a block of (p, kappa) at a time when
   // it exceeds max(this many elements, the largest environment)
   int rdm_block_elems = 1 << 24;
+  // device address of this device's copy of `scratch` (hipGetSymbolAddress at creation)
+  void* scratch = nullptr;
   hipmps_engine_s* big = nullptr;
   std::thread big_worker;
   std::mutex big_mu;
@@ -268,6 +277,27 @@ struct PendingCircuit {
 };
 
 namespace {
+
+// hipGetSymbolAddress of `scratch`, once per device (the device current), kept per device
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+int engine_scratch_setup(hipmps_engine e) {
+  if (e->device >= SCRATCH_MAX_DEVICES) return 0;  // the no-alloc entry point refuses such a device
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[e->device]) {
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      return fail(HIPMPS_E_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    }
+    g_scratch_ptr[e->device] = p;
+  }
+  e->scratch = g_scratch_ptr[e->device];
+  return 0;
+}
 
 int set_device(hipmps_engine e) {
   HIP_TRY(hipSetDevice(e->device));
@@ -2084,6 +2114,11 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
     (void)hipGetLastError();
     e->jacobi_lanes = 1;
   }
+  if (const int rc = engine_scratch_setup(e)) {
+    const std::string msg = t_err;
+    (void)hipmps_engine_destroy(e);
+    return fail(rc, msg);
+  }
   *out = e;
   return 0;
 }
@@ -2471,6 +2506,75 @@ int hipmps_amplitudes(hipmps_state s, const int8_t* bits, int k, double* out) {
     out[2 * t] = v[t].x;
     out[2 * t + 1] = v[t].y;
   }
+  return 0;
+}
+
+// hipmps_amplitudes with every intermediate in this device's copy of `scratch`: no device or
+// host allocation, the same kernel on the same inputs (results equal bit for bit).  Layout in
+// the scratch: n chain descriptors | k x n bits | two k x max(chi) chain vectors.
+int hipmps_amplitudes_noalloc(hipmps_state s, const int8_t* bits, int k, double* out) {
+  if (!s || !bits || !out || k <= 0) return fail(HIPMPS_E_ARG, "bad arguments");
+  hipmps_engine E = s->e;
+  if (!E->scratch) return fail(HIPMPS_E_UNSUPPORTED, "no static scratch on this device");
+  if (is_purified(s))
+    return fail(HIPMPS_E_UNSUPPORTED, "amplitudes are not defined for a purified (mixed) state");
+  bool queued = !s->wave.empty() || !s->cwave.empty();
+  for (int q = 0; q < s->n; ++q) queued = queued || s->has_pend[q];
+  if (queued)
+    return fail(HIPMPS_E_ARG, "the state has queued gates; hipmps_flush it first (a flush may grow site tensors)");
+  const int n = s->n;
+  for (size_t t = 0; t < (size_t)k * n; ++t)
+    if (bits[t] != 0 && bits[t] != 1) return fail(HIPMPS_E_ARG, "bits must be 0 or 1");
+  int maxchi = 1;
+  for (int b = 0; b <= n; ++b) maxchi = std::max(maxchi, s->chi[b]);
+  const size_t desc_b = align_up((size_t)n * sizeof(ChainTask));
+  const size_t bits_b = align_up((size_t)k * n);
+  const size_t vec_b = align_up((size_t)k * maxchi * sizeof(double2));
+  const size_t need = desc_b + bits_b + 2 * vec_b;
+  if (need > HIPMPS_NOALLOC_SCRATCH_BYTES)
+    return fail(HIPMPS_E_NOMEM, "hipmps_amplitudes_noalloc needs " + std::to_string(need) + " bytes for " +
+                                    std::to_string(k) + " bit strings at bond " + std::to_string(maxchi) +
+                                    "; the static scratch holds " + std::to_string(HIPMPS_NOALLOC_SCRATCH_BYTES));
+  RC_TRY(set_device(E));
+  char* base = (char*)E->scratch;
+  ChainTask* d_desc = (ChainTask*)base;
+  int8_t* d_bits = (int8_t*)(base + desc_b);
+  double2* v0 = (double2*)(base + desc_b + bits_b);
+  double2* v1 = (double2*)(base + desc_b + bits_b + vec_b);
+  // descriptors of all sites, a block at a time through a stack buffer (synchronous copies)
+  constexpr int BLK = 32;
+  ChainTask h[BLK];
+  for (int k0 = 0; k0 < n; k0 += BLK) {
+    const int m = std::min(BLK, n - k0);
+    for (int j = 0; j < m; ++j) {
+      const int site = k0 + j;
+      ChainTask& c = h[j];
+      memset(&c, 0, sizeof(c));
+      c.vin = (site & 1) ? v1 : v0;
+      c.vout = (site & 1) ? v0 : v1;
+      c.A = s->sites[site].p;
+      c.cl = s->chi[site];
+      c.cr = s->chi[site + 1];
+      c.rows_in = k;
+      c.rows_out = k;
+      c.bits = d_bits;
+      c.n = n;
+      c.site = site;
+      c.mode = 0;
+      c.block_begin = 0;
+      c.blocks = k * ((c.cr + 255) / 256);
+    }
+    HIP_TRY(hipMemcpy(d_desc + k0, h, (size_t)m * sizeof(ChainTask), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(d_bits, bits, (size_t)k * n, hipMemcpyHostToDevice));
+  HIP_TRY(launch_fill_c(v0, (size_t)k, make_double2(1.0, 0.0), E->stream));  // rows of ones (chi_0 = 1)
+  for (int site = 0; site < n; ++site) {
+    const int blocks = k * ((s->chi[site + 1] + 255) / 256);
+    HIP_TRY(launch_chain(d_desc + site, 1, blocks, s->fp32, E->stream));
+  }
+  const double2* res = (n & 1) ? v1 : v0;
+  HIP_TRY(hipMemcpyAsync(out, res, (size_t)k * sizeof(double2), hipMemcpyDeviceToHost, E->stream));
+  HIP_TRY(hipStreamSynchronize(E->stream));
   return 0;
 }
 

@@ -170,6 +170,8 @@ hipError_t launch_trace_prod(const double2* const* d_a, const double2* const* d_
 hipError_t launch_check_finite(const void* p, size_t n_complex, int fp32, int* d_flag,
                                hipStream_t s);
 hipError_t launch_fill_identity1(double2* p, hipStream_t s);
+// p[i] = v for i < n
+hipError_t launch_fill_c(double2* p, size_t n, double2 v, hipStream_t s);
 // reduced density matrix of k <= 15 qubits: E[A] (4^k labels, base-4 digit t = 2 b_t + b'_t)
 // scattered to out[x + 2^k x'], x = sum_t b_t 2^pos[t], x' = sum_t b'_t 2^pos[t]
 struct RdmPerm {

@@ -1077,6 +1077,16 @@ hipError_t launch_fill_identity1(double2* p, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_fill_c(double2* p, size_t n, double2 v) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+hipError_t launch_fill_c(double2* p, size_t n, double2 v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_c, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 // density-matrix readout: label A holds, in base-4 digit t, the ket (2s) and bra (1s) component
 // of the t-th measured site; element A goes to out[x + 2^k x'] with x, x' the components placed
 // at the bits P.pos[t]

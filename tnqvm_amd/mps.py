@@ -292,6 +292,15 @@ class MpsState:
                                       out.view(np.float64).ctypes.data_as(_dp)))
         return out
 
+    def amplitudes_noalloc(self, bits) -> np.ndarray:
+        """amplitudes() through hipmps_amplitudes_noalloc (static device scratch, no allocation);
+        the state must be flushed first."""
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8))
+        out = np.zeros(b.shape[0], dtype=np.complex128)
+        check(lib().hipmps_amplitudes_noalloc(self.h, b.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), b.shape[0],
+                                              out.view(np.float64).ctypes.data_as(_dp)))
+        return out
+
     def slice(self, bits) -> np.ndarray:
         b = np.ascontiguousarray(np.asarray(bits, dtype=np.int32))
         nopen = int((b < 0).sum())
