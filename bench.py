@@ -2,12 +2,16 @@
 """hip-mps headline benchmark: 2-qubit-gate applies/s at chi = 256, with circuit
 latency p50/p99, batched mixed precision against the plain per-circuit FP64 path.
 
-Workload (synthetic, seeded): 40-qubit brickwork of Haar-random gates (a Haar 2x2 on
-every qubit, then Haar 4x4 gates on alternating even/odd neighbour pairs), depth 40,
-chi_max = 256, svd-cutoff 1e-10.  A "step" is one brickwork layer advanced for every
-circuit of the batch.  The first --warmup layers (default 12) grow the middle bonds to
-256 and are not timed; the remaining --steps layers (default 28) run at chi = 256 on
-the interior bonds (32 of the 39 bonds) and are timed.
+Headline workload (synthetic, seeded): 8 circuits of the 40-qubit brickwork of Haar-random
+gates (a Haar 2x2 on every qubit, then Haar 4x4 gates on alternating even/odd neighbour
+pairs), depth 40, chi_max = 256, svd-cutoff 1e-10.  A "step" is one brickwork layer
+advanced for every circuit of the batch.  The first layers (at least 12; 40 - 1 - warmup -
+steps when that is more) run the full brickwork and bring bonds 8..32 to chi = 256.  The
+--warmup and --steps layers that follow apply the brickwork's Haar gates only on the pairs
+(i, i+1) with 8 <= i <= 30, whose theta' is 512 x 512 once those bonds are saturated: every
+timed apply is theta 512 x 512 -> SVD -> clamp to 256, the unit of the headline metric.
+The engine counts the timed applies by theta' size; `value` counts only the 512 x 512 ones
+over the whole timed span.  The remaining layers (at least one) run untimed.
 
 Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
   plain_fp64_value   the same layers, one circuit at a time, complex128 everywhere
@@ -20,7 +24,8 @@ Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
                      (chi_max 1024) through the continuous-batching loop and through the
                      plain path one at a time: circuits/s and p50/p99 latency for both
   latency_ms_p50/p99 the batched serving latencies (arrival to result)
-  roofline           k_jacobi_round (the dominant kernel), FP64 MFMA bound
+  roofline           the SVD (double LQ + Jacobi rounds) of the timed applies, FP64 bound:
+                     nominal 704 chi^3 flops per apply over its HIP-event time
   noisy              the purified-MPS (noisy) path: a 40-qubit u3/CNOT circuit of depth 4 with the
                      ibmqx2 backend's single-qubit channels after every gate (qubit q takes the
                      channels of backend qubit q mod 5), no bond cap, cut 1e-9: wall time per
@@ -56,36 +61,66 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def layer_2q_counts(depth):
-    return [len(range(l % 2, N_QUBITS - 1, 2)) for l in range(depth)]
+DEPTH = 40
+SAT_MIN = 12          # full brickwork layers that saturate bonds 8..32 of the 40-qubit chain at 256
+PAIR_LO, PAIR_HI = 8, 30  # pairs (i, i+1) whose theta' is (2 chi_i) x (2 chi_(i+2)) = 512 x 512 then
+RANK_512 = 9          # engine histogram bucket of min(m, n) in [512, 1024)
+NOMINAL_SVD = 704.0   # 4 (14 m n^2 + 8 n^3) at m = n = 2 chi, over chi^3
+NOMINAL_APPLY = 736.0  # theta (32 chi^3) + nominal SVD; the gate term (128 chi^2) is below 1%
+
+
+def headline_plan(warmup, steps):
+    """Layers of the headline circuits: (saturation, warm, timed, tail) counts."""
+    sat = max(SAT_MIN, DEPTH - 1 - warmup - steps)
+    depth = max(DEPTH, sat + warmup + steps + 1)
+    return sat, warmup, steps, depth - sat - warmup - steps
+
+
+def interior(layer):
+    """The gates of one brickwork layer on the pairs PAIR_LO..PAIR_HI (and the 1-qubit gates on
+    their qubits, which fuse into them)."""
+    keep2 = [g for g in layer if len(g["qubits"]) == 2 and PAIR_LO <= min(g["qubits"]) <= PAIR_HI]
+    qs = {q for g in keep2 for q in g["qubits"]}
+    return [g for g in layer if len(g["qubits"]) == 1 and g["qubits"][0] in qs] + keep2
+
+
+def headline_circuit(seed, warmup, steps):
+    sat, warm, timed, tail = headline_plan(warmup, steps)
+    layers = C.brickwork_layers(C.haar_brickwork(N_QUBITS, sat + warm + timed + tail, seed=seed), N_QUBITS)
+    out = [g for l in layers[:sat] for g in l]
+    out += [g for l in layers[sat:sat + warm + timed] for g in interior(l)]
+    out += [g for l in layers[sat + warm + timed:] for g in l]
+    return out
 
 
 def run_layers(eng, circuits, precision, warmup, steps):
-    """Advance circuits together through the serving loop; time the last `steps` layers."""
+    """Advance circuits together through the serving loop (one layer per step); time the
+    `steps` interior layers after the saturation and warm layers."""
+    sat, warm, timed, _ = headline_plan(warmup, steps)
     srv = mps.Server(eng)
     tr = mps.trunc_opts(cutoff=CUTOFF, max_bond=CHI, precision=precision)
     tickets = [srv.submit(N_QUBITS, c, tr) for c in circuits]
-    for _ in range(warmup):
+    for _ in range(sat + warm):
         srv.step()
     eng.profile(reset=True)
     step_ms = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(timed):
         ts = time.perf_counter()
         srv.step()
         step_ms.append(1e3 * (time.perf_counter() - ts))
     total = time.perf_counter() - t0
     prof = eng.profile(reset=True)
-    # drain (measurements of the final state belong to the last step already)
     results = []
-    for t in tickets:
+    for t in tickets:  # the tail layers and the measurements, untimed
         r = srv.poll(t)
         while r is None:
             srv.step()
             r = srv.poll(t)
         results.append(r)
-    applies = sum(layer_2q_counts(warmup + steps)[warmup:]) * len(circuits)
-    return applies, total, step_ms, prof, results
+    hist = prof["applies_by_rank"]
+    applies = hist[RANK_512]
+    return applies, sum(hist) - applies, total, step_ms, prof, results
 
 
 def uneven_batch(count, seed):
@@ -185,17 +220,20 @@ def fp64_sensitivity(eng, job, ez_ref):
     return float(np.abs(ez - ez_ref).max())
 
 
-def gate_at_a_time(eng, circuit, warmup, steps, max_applies=40):
+def gate_at_a_time(eng, seed, warmup, steps, max_applies=40):
     """Plain FP64, one 2-qubit gate per wave (the reference visitor's gate-by-gate model):
-    the circuit is brought to layer `warmup` normally, then 2-qubit gates of the timed layers
-    are applied and flushed one at a time; a bounded sample of `max_applies` gates is timed."""
+    the circuit is brought through its saturation and warm layers normally, then 2-qubit
+    gates of the timed (interior) layers are applied and flushed one at a time; a bounded
+    sample of `max_applies` gates is timed."""
+    sat, warm, timed, tail = headline_plan(warmup, steps)
     s = mps.MpsState(eng, N_QUBITS, cutoff=CUTOFF, max_bond=CHI, precision="fp64")
-    per_layer = C.brickwork_layers(circuit, N_QUBITS)
-    for l in range(warmup):
+    layers = C.brickwork_layers(C.haar_brickwork(N_QUBITS, sat + warm + timed + tail, seed=seed), N_QUBITS)
+    per_layer = layers[:sat] + [interior(l) for l in layers[sat:]]
+    for l in range(sat + warm):
         s.run(per_layer[l])
     s.flush()
     done, t0 = 0, time.perf_counter()
-    for l in range(warmup, warmup + steps):
+    for l in range(sat + warm, sat + warm + timed):
         for g in per_layer[l]:
             s.run([g])
             if len(g["qubits"]) == 2:
@@ -329,8 +367,9 @@ def calibrate_copy(nbytes: int, reps: int = 4):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=28, help="timed brickwork layers")
-    ap.add_argument("--warmup", type=int, default=12, help="untimed layers (bond growth to chi=256)")
+    ap.add_argument("--steps", type=int, default=20, help="timed interior layers (every apply chi = 256)")
+    ap.add_argument("--warmup", type=int, default=7,
+                    help="untimed interior layers after the (at least 12) saturation layers")
     ap.add_argument("--batch", type=int, default=8, help="circuits advanced together")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-plain", action="store_true")
@@ -366,65 +405,86 @@ def main():
               "across GPUs with no communication, so the per-GPU figure at N=1 is the measurement.",
               file=sys.stderr)
         sys.exit(2)
-    depth = args.warmup + args.steps
+    sat, warm, timed, tail = headline_plan(args.warmup, args.steps)
+    depth = sat + warm + timed + tail
     eng = mps.Engine()
     if args.lq_min_rank >= 0:
         eng.set_lq_precondition(args.lq_min_rank)
     eng.set_timing(True)
 
-    log(f"batched {args.precision}: {args.batch} x 40-qubit depth-{depth} Haar brickwork, chi_max {CHI}")
-    circuits = [C.haar_brickwork(N_QUBITS, depth, seed=0x7E5A0002 + k) for k in range(args.batch)]
-    applies, total, step_ms, prof, res = run_layers(eng, circuits, args.precision, args.warmup, args.steps)
+    log(f"batched {args.precision}: {args.batch} x 40-qubit depth-{depth} Haar brickwork, chi_max {CHI}; "
+        f"{sat} saturation + {warm} warm layers, {timed} timed interior layers")
+    seeds = [0x7E5A0002 + k for k in range(args.batch)]
+    circuits = [headline_circuit(sd, args.warmup, args.steps) for sd in seeds]
+    applies, other, total, step_ms, prof, res = run_layers(eng, circuits, args.precision, args.warmup, args.steps)
     value = applies / total
-    log(f"  {applies} applies in {total:.3f} s -> {value:.1f} applies/s; jacobi {prof['ms_jacobi']:.0f} ms, "
-        f"rounds {prof['ms_jacobi_rounds']:.0f} ms, launches {prof['jacobi_launches']}")
+    svd_ms = prof["ms_lq"] + prof["ms_jacobi"]
+    log(f"  {applies} applies at theta 512x512 (+{other} other) in {total:.3f} s -> {value:.1f} applies/s; "
+        f"theta {prof['ms_theta']:.0f} ms, lq {prof['ms_lq']:.0f} ms, jacobi {prof['ms_jacobi']:.0f} ms "
+        f"(rounds {prof['ms_jacobi_rounds']:.0f} ms, {prof['jacobi_launches']} launches), split {prof['ms_split']:.0f} ms")
 
-    # roofline of the dominant kernel (k_jacobi_round): executed MFMA flops / the HIP-event span of
-    # its rounds (the SVDs of a wave are dealt to two streams whose launches overlap, so the
-    # span is the union of the launch intervals, and span / launches is the time per launch)
-    jac_tflops = prof["jacobi_flops"] / max(prof["ms_jacobi_rounds"], 1e-9) / 1e9
+    # roofline of the SVD (double LQ + Jacobi rounds + reflectors back), the dominant phase:
+    # nominal 704 chi^3 flops per chi = 256 apply (the Golub-Reinsch count for a 512 x 512
+    # complex SVD, fixed by convention so that fewer executed flops count as faster) over the
+    # HIP-event time of that phase on the engine stream (its two lanes joined)
+    svd_nominal = applies * NOMINAL_SVD * CHI ** 3
+    svd_tflops = svd_nominal / max(svd_ms, 1e-9) / 1e9
+    waves = max(prof["waves"], 1)
+    # executed flops of k_jacobi_round (counted on the device) over the union span of its launches
+    jac_exec_tflops = prof["jacobi_flops"] / max(prof["ms_jacobi_rounds"], 1e-9) / 1e9
     avg_launch_us = 1e3 * prof["ms_jacobi_rounds"] / max(prof["jacobi_launches"], 1)
-    # HBM bytes per launch: per-block-pair FETCH/WRITE from the committed rocprofv3 counter runs
-    # (profiles/k_jacobi_round_traffic.json) times the block pairs this run launched per dispatch
-    traffic = None
+    # HBM bytes per k_jacobi_round launch: per-block-pair FETCH/WRITE from the committed rocprofv3
+    # counter runs times the block pairs this run launched per dispatch; against the algorithmic
+    # bytes of the SVDs (read and write the 512 x 512 complex128 matrix once: 2 m n 16)
+    traffic, traffic_ratio = None, None
     tpath = os.path.join(ROOT, "profiles", "k_jacobi_round_traffic.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
-        pairs_per_launch = prof["jacobi_pairs_visited"] / max(prof["jacobi_launches"], 1)
-        traffic = round((tj["fetch_bytes_per_pair"] + tj["write_bytes_per_pair"]) * pairs_per_launch)
-    roofline = {"bound": "mfma", "achieved": round(jac_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(jac_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_jacobi_round", "avg_launch_us": round(avg_launch_us, 2),
-                "timing": "union span of the rounds on two streams / launches",
+        per_pair = tj["fetch_bytes_per_pair"] + tj["write_bytes_per_pair"]
+        traffic = round(per_pair * prof["jacobi_pairs_visited"] / max(prof["jacobi_launches"], 1))
+        traffic_ratio = round(per_pair * prof["jacobi_pairs_visited"] / max(1.0, applies * 2 * 512 * 512 * 16), 2)
+    roofline = {"bound": "mfma", "achieved": round(svd_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(svd_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                "kernel": "SVD of theta' 512x512: k_lq_panel0/k_lq_update (double LQ), k_jacobi_round, k_lq_apply_q",
+                "flops_per_unit": NOMINAL_SVD * CHI ** 3, "unit_of_work": "one chi=256 apply's SVD (nominal 704 chi^3)",
+                "avg_wave_ms": round(svd_ms / waves, 3), "svds_per_wave": round(applies / waves, 1),
+                "timing": "HIP events on the engine stream around the LQ and Jacobi phases of each wave",
+                "jacobi_round": {"executed_tflops": round(jac_exec_tflops, 3),
+                                 "executed_over_nominal": round(prof["jacobi_flops"] / max(svd_nominal, 1.0), 3),
+                                 "avg_launch_us": round(avg_launch_us, 2), "launches": prof["jacobi_launches"],
+                                 "traffic_bytes_per_launch": traffic, "traffic_over_algorithmic": traffic_ratio},
                 "precision_note": "mixed path: the Gram of the first 5 sweeps of large SVDs on FP32 MFMA; rotations, "
-                                  "row updates and the final sweeps on FP64 MFMA; peak is the FP64 one",
-                "launches": prof["jacobi_launches"],
-                "flops_per_launch": prof["jacobi_flops"] / max(prof["jacobi_launches"], 1)}
-    nominal_flops = 736.0 * CHI ** 3  # per apply at chi=256 (theta + gate + nominal SVD)
+                                  "row updates and the final sweeps on FP64 MFMA; peak is the FP64 one"}
+    nominal_flops = NOMINAL_APPLY * CHI ** 3
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "2q-gate applies/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * total / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * total / max(timed, 1), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64+f64" if args.precision == "mixed" else "c128",
         "data": "synthetic",
-        "config": {"workload": f"40-qubit Haar brickwork, depth {depth}, chi_max {CHI}, svd-cutoff {CUTOFF:g}; "
-                               f"timed layers {args.warmup}-{depth} (chi=256 on 32 of 39 bonds)",
+        "config": {"workload": f"{args.batch} x 40-qubit Haar brickwork, depth {depth}, chi_max {CHI}, svd-cutoff "
+                               f"{CUTOFF:g}; layers 0-{sat - 1} full (bonds 8-32, 25 of 39, saturate at 256), then "
+                               f"{warm} warm + {timed} timed layers of the brickwork's gates on pairs "
+                               f"{PAIR_LO}-{PAIR_HI} only (every timed apply theta 512x512), {tail} full tail layer(s)",
                    "batch": args.batch, "precision": args.precision,
-                   "precision_detail": "complex64 sites and theta, FP64 Jacobi SVD/norms" if args.precision == "mixed"
-                   else "complex128 everywhere"},
+                   "precision_detail": "complex64 sites and theta (FP32 MFMA), FP64 Jacobi SVD, norms and weights"
+                   if args.precision == "mixed" else "complex128 everywhere"},
+        "timed_applies": {"theta_512x512": applies, "other": other},
         "nominal_tflops": round(value * nominal_flops / 1e12, 3),
         "nominal_frac_fp64": round(value * nominal_flops / 1e12 / PEAK_FP64_TFLOPS, 4),
         "fidelity_est": float(np.mean([r["fidelity_est"] for r in res])),
         "roofline": roofline,
+        "phase_ms": {k: round(prof[k], 2) for k in ("ms_theta", "ms_lq", "ms_jacobi", "ms_split")},
         "jacobi_sweeps_per_svd_chi256": round(prof["sweeps_large"] / max(1, prof["svds_large"]), 2),
     }
     if not args.no_plain:
         log("plain fp64, one circuit at a time (bonds of a layer batched)")
-        pa, pt, _, pprof, _ = run_layers(eng, circuits[:1], "fp64", args.warmup, args.steps)
+        pa, po, pt, _, pprof, _ = run_layers(eng, circuits[:1], "fp64", args.warmup, args.steps)
         out["plain_fp64_value"] = round(pa / pt, 2)
         out["speedup_vs_plain"] = round(value / (pa / pt), 3)
-        log(f"  {pa} applies in {pt:.3f} s -> {pa / pt:.1f} applies/s")
+        log(f"  {pa} applies (+{po} other) in {pt:.3f} s -> {pa / pt:.1f} applies/s")
         log("plain fp64, one circuit, one 2-qubit gate at a time")
-        ga, gt = gate_at_a_time(eng, circuits[0], args.warmup, args.steps)
+        ga, gt = gate_at_a_time(eng, seeds[0], args.warmup, args.steps)
         out["plain_fp64_gate_at_a_time_value"] = round(ga / gt, 2)
         out["speedup_vs_gate_at_a_time"] = round(value / (ga / gt), 3)
         log(f"  {ga} applies in {gt:.3f} s -> {ga / gt:.1f} applies/s")

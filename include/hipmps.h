@@ -268,6 +268,12 @@ typedef struct {
   long long jacobi_sweeps; /* sweeps summed over all SVDs */
   long long svds_large;    /* SVDs with min(m, n) >= 256 */
   long long sweeps_large;  /* sweeps summed over those */
+  double ms_lq;            /* HIP-event time of the double-LQ factorisations (between theta and
+                              the Jacobi sweeps); ms_jacobi holds the sweeps and the application
+                              of the LQ reflectors back to the rows */
+  long long applies_by_rank[14]; /* 2-qubit applies by floor(log2(min(m, n))) of theta' (m x n):
+                                    index 9 counts 512 <= min(m, n) < 1024, i.e. theta' 512 x 512
+                                    when chi_max = 256 */
 } hipmps_profile_t;
 int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset);
 int hipmps_engine_set_timing(hipmps_engine e, int enabled);

@@ -100,7 +100,8 @@ class Profile(ctypes.Structure):
                 ("jacobi_pairs_visited", ctypes.c_longlong), ("jacobi_flops", ctypes.c_double),
                 ("waves", ctypes.c_longlong), ("wave_tasks", ctypes.c_longlong),
                 ("jacobi_sweeps", ctypes.c_longlong), ("svds_large", ctypes.c_longlong),
-                ("sweeps_large", ctypes.c_longlong)]
+                ("sweeps_large", ctypes.c_longlong), ("ms_lq", ctypes.c_double),
+                ("applies_by_rank", ctypes.c_longlong * 14)]
 
 
 _lib = None

@@ -29,6 +29,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -181,6 +182,7 @@ struct hipmps_engine_s {
   double tol = 1e-13;       // FP64 path: relative orthogonality of converged rows
   double tol_mixed = 1e-8;  // mixed path: factors are stored in complex64
   int inner_sweeps = 1;
+  int debug_sweeps = 0;  // HIPMPS_DEBUG_SWEEPS: per-sweep rotated / visited block pairs on stderr
   int lq_min_rank = 64;  // Jacobi on the double-LQ factor when min(m, n) >= this (0: never)
   DevBuf ws;     // per-wave arena
   DevBuf desc;   // task descriptors
@@ -191,7 +193,7 @@ struct hipmps_engine_s {
   std::mutex mu;
   bool timing = false;
   hipmps_profile_t prof{};
-  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // serving
   uint64_t next_ticket = 1;
   std::deque<PendingCircuit*> queue;
@@ -776,6 +778,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   if (n64 > 0) HIP_TRY(launch_theta(dth, n64, tile_acc[0], 0, st));
   if (N - n64 > 0) HIP_TRY(launch_theta(dth + n64, N - n64, tile_acc[1], 1, st));
   HIP_TRY(launch_jacobi_prep(dsp, djt, N, max_rpad, max_c, st));
+  if (E->timing) HIP_TRY(hipEventRecord(E->ev[6], st));
   // LQ lanes: with many items, contiguous slices of the LQ task arrays run their panel
   // sequences on separate streams, so that one slice's launch tails overlap another's work
   const int lq_lanes = nlq >= 64 ? std::max(1, std::min(E->jacobi_lanes, nlq)) : 1;
@@ -895,6 +898,12 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       (void)hipEventElapsedTime(&ms, E->ev[4], E->ev[5]);
       E->prof.ms_jacobi_rounds += ms;
     }
+    if (E->debug_sweeps) {
+      long long R = 0, V = 0;
+      for (int a = 0; a < nact; ++a) R += hcounts[2 * active[a]], V += hcounts[2 * active[a] + 1];
+      fprintf(stderr, "[hipmps sweep] wave %lld sweep %d active %d rotated %lld visited %lld\n", E->prof.waves, sweep,
+              nact, R, V);
+    }
     std::vector<int> next;
     for (int a = 0; a < nact; ++a) {
       int q = active[a];
@@ -961,16 +970,23 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[3], st));
   HIP_TRY(hipStreamSynchronize(st));
   if (E->timing) {
-    float a = 0, b = 0, c = 0;
-    (void)hipEventElapsedTime(&a, E->ev[0], E->ev[1]);
+    float a = 0, b = 0, c = 0, d = 0;
+    (void)hipEventElapsedTime(&a, E->ev[0], E->ev[6]);
+    (void)hipEventElapsedTime(&d, E->ev[6], E->ev[1]);
     (void)hipEventElapsedTime(&b, E->ev[1], E->ev[2]);
     (void)hipEventElapsedTime(&c, E->ev[2], E->ev[3]);
     E->prof.ms_theta += a;
+    E->prof.ms_lq += d;
     E->prof.ms_jacobi += b;
     E->prof.ms_split += c;
   }
   E->prof.waves += 1;
   E->prof.wave_tasks += N;
+  for (int q = 0; q < N; ++q) {
+    int lg = 0;
+    while (lg < 13 && (2 << lg) <= items[q].r) ++lg;
+    E->prof.applies_by_rank[lg] += 1;
+  }
   for (int q = 0; q < N; ++q) {
     E->prof.jacobi_sweeps += sweeps_used[q];
     if (items[q].r >= 256) {
@@ -2103,6 +2119,7 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
   if (opts && opts->jacobi_tol_mixed > 0) e->tol_mixed = opts->jacobi_tol_mixed;
   for (auto& ev : e->ev) (void)hipEventCreate(&ev);
   if (const char* v = getenv("HIPMPS_GRAM32_SWEEPS")) e->gram32_sweeps = std::max(0, atoi(v));
+  if (const char* v = getenv("HIPMPS_DEBUG_SWEEPS")) e->debug_sweeps = atoi(v);
   if (const char* v = getenv("HIPMPS_JACOBI_LANES"))
     e->jacobi_lanes = std::max(1, std::min(hipmps_engine_s::MAX_LANES, atoi(v)));
   e->lane_stream[0] = e->stream;

@@ -118,7 +118,9 @@ class Engine:
     def profile(self, reset: bool = False) -> dict:
         p = C.Profile()
         check(lib().hipmps_engine_profile(self.h, ctypes.byref(p), int(reset)))
-        return {k: getattr(p, k) for k, _ in C.Profile._fields_}
+        out = {k: getattr(p, k) for k, _ in C.Profile._fields_}
+        out["applies_by_rank"] = list(out["applies_by_rank"])
+        return out
 
     # -- batches -------------------------------------------------------------------
     def _circuit(self, n, gates, trunc, zz):
