@@ -347,6 +347,36 @@ def cpu_baseline_noisy(ops, ez_gpu, seconds=3.0):
             "max_abs_dz_vs_gpu": float(np.abs(ez - ez_gpu).max())}
 
 
+def measure_kernels(precision: str, amps: int = 1024):
+    """Workload of the per-kernel counter runs of the memory-bound kernels: one 40-qubit Haar
+    brickwork state brought to saturation (12 full layers, bonds 8..32 at chi 256), then a Haar
+    1-qubit gate on every qubit and a flush (k_apply_1q on every site), `amps` bit-string
+    amplitudes (k_chain on every site), <Z_i> of every qubit (environment GEMMs) and the norm."""
+    eng = mps.Engine()
+    rng = np.random.default_rng(0x7E5A0040)
+    s = mps.MpsState(eng, N_QUBITS, cutoff=CUTOFF, max_bond=CHI, precision=precision)
+    s.run(C.haar_brickwork(N_QUBITS, SAT_MIN, seed=0x7E5A0041))
+    s.flush()
+    one = [{"name": "unitary", "qubits": (q,), "matrix": C.haar_unitary(rng, 2)} for q in range(N_QUBITS)]
+    t0 = time.perf_counter()
+    s.run(one)
+    s.flush()
+    t1 = time.perf_counter()
+    bits = C.random_bitstrings(N_QUBITS, amps, seed=0x7E5A0042)
+    amp = s.amplitudes(bits)
+    t2 = time.perf_counter()
+    ez = s.expval_z_all()
+    nrm = s.norm()
+    t3 = time.perf_counter()
+    out = {"precision": precision, "bonds": [int(b) for b in s.bonds()], "amplitudes": amps,
+           "ms_1q_all_sites": round(1e3 * (t1 - t0), 3), "ms_amplitudes": round(1e3 * (t2 - t1), 3),
+           "ms_expval_z_all_and_norm": round(1e3 * (t3 - t2), 3), "norm": nrm,
+           "checksum": float(np.abs(amp).sum() + np.abs(ez).sum())}
+    s.close()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
 def calibrate_copy(nbytes: int, reps: int = 4):
     """Plain device-to-device copies of a known size, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE readings (run under the profiler: bench.py --calibrate-copy BYTES)."""
@@ -390,9 +420,15 @@ def main():
                     help="only run plain device copies of BYTES (profiler counter calibration)")
     ap.add_argument("--noisy-only", action="store_true",
                     help="only the noisy block (for profiling the purified-MPS path)")
+    ap.add_argument("--measure-kernels", action="store_true",
+                    help="only the workload of the memory-bound kernels (1q gates, amplitudes, <Z>) on a "
+                         "saturated chi=256 state, for counter runs")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.measure_kernels:
+        measure_kernels(args.precision)
         return
     if args.noisy_only:
         eng = mps.Engine()
