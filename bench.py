@@ -248,6 +248,66 @@ def gate_at_a_time(eng, seed, warmup, steps, max_applies=40):
     return done, el
 
 
+def cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_baseline_lapack(threads, seconds=8.0):
+    """The LAPACK-class CPU comparator (oracle/lapack_mps.py: the exatn-mps step with numpy's zgesdd
+    for the SVD) at chi = 256, Haar gates on the two middle bonds of an 18-site chain whose bonds are
+    min(2^k, 2^(18-k), 256), with the BLAS/LAPACK thread pool limited to `threads`."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle.lapack_mps import LapackMps
+    n = 18
+    bonds = [min(2 ** min(k, n - k), CHI) for k in range(n + 1)]
+    rng = np.random.default_rng(0xC0FFEE)
+    m = LapackMps(n, cutoff=CUTOFF, max_bond=CHI)
+    for k in range(n):
+        a = (rng.standard_normal((bonds[k], 2, bonds[k + 1])) + 1j * rng.standard_normal((bonds[k], 2, bonds[k + 1])))
+        m.set_site(k, a / np.sqrt(2 * bonds[k + 1]))
+    applies = 0
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        while True:
+            q = 8 + (applies % 2)
+            m.apply_2q(q, q + 1, C.haar_unitary(rng, 4))
+            applies += 1
+            el = time.perf_counter() - t0
+            if (el >= seconds and applies >= 2) or applies >= 400:
+                break
+    return {"value": applies / el, "unit": "2q-gate applies/s @chi=256", "cores": threads, "kind": "port",
+            "sample": f"{applies} Haar 2q applies on the middle bonds (theta 512x512, chi_max 256) of an 18-site "
+                      f"FP64 MPS, {el:.1f} s, oracle/lapack_mps.py (numpy zgesdd, {threads} BLAS thread(s))"}
+
+
+def oracle_check_sample(sample, threads, budget_s=25.0):
+    """Per-result error of the uneven batch against the FP64 oracle (oracle/mps_oracle.cpp, the same
+    truncation), on the seeded sample chosen in the uneven block, cheapest first, within a time budget."""
+    from oracle import oracle as O
+    out, t0 = [], time.perf_counter()
+    for rec in sample:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        job = rec["job"]
+        m = O.Mps(job["n"], cutoff=CUTOFF, max_bond=job["chi"], nthreads=threads).run(job["gates"])
+        z = np.array([m.expval_z_parity([q]) for q in range(job["n"])])
+        out.append({"qubits": job["n"], "chi_max": job["chi"], "two_qubit_gates": C.count_2q(job["gates"]),
+                    "fidelity_est": rec["fidelity_est"], "bond_equal": int(max(m.bonds())) == rec["max_bond"],
+                    "z_error_mixed": float(np.abs(rec["ez_mixed"] - z).max()),
+                    "z_error_plain_fp64": float(np.abs(rec["ez_fp64"] - z).max()) if rec["ez_fp64"] is not None else None,
+                    "fp64_sensitivity": rec["fp64_sensitivity"]})
+    return out, time.perf_counter() - t0
+
+
 def cpu_baseline(seconds=12.0):
     """FP64 oracle (CPU restatement of the exatn-mps algorithm) at chi = 256: Haar gates on the
     two middle bonds of an 18-site chain whose bonds are min(2^k, 2^(18-k), 256)."""
@@ -271,6 +331,37 @@ def cpu_baseline(seconds=12.0):
     return {"value": applies / el, "unit": "2q-gate applies/s @chi=256", "cores": threads, "kind": "port",
             "sample": f"{applies} Haar 2q applies on the middle bonds (theta 512x512, chi_max 256) of an "
                       f"18-site FP64 MPS, {el:.1f} s, oracle/mps_oracle.cpp with {threads} OpenMP threads"}
+
+
+def qaoa_60(eng, precision):
+    """60-qubit QAOA MaxCut on a seeded random 3-regular graph (90 edges), p = 4, routed by the
+    swap network (7080 2-qubit applies), chi_max 512, svd-cutoff 1e-12, one circuit alone: wall
+    time of the gates, the energy sum_edges <Z_i Z_j> unnormalised (the reference's exp-val
+    semantics: no renormalisation after truncation) and divided by <psi|psi>."""
+    gates, edges, pos, _ = C.qaoa_maxcut(60, 4, seed=0x7E5A0003)
+    pairs = [(pos[i], pos[j]) for i, j in edges]
+    eng.profile(reset=True)
+    t0 = time.perf_counter()
+    s = mps.MpsState(eng, 60, cutoff=1e-12, max_bond=512, precision=precision).run(gates)
+    s.flush()
+    dt = time.perf_counter() - t0
+    zz = s.expval_zz(pairs)
+    nrm = s.norm()
+    st = s.stats()
+    prof = eng.profile(reset=True)
+    s.close()
+    applies = C.count_2q(gates)
+    e = float(zz.sum())
+    return {"precision": precision, "seconds": round(dt, 3), "applies": applies,
+            "applies_per_s": round(applies / dt, 2), "applies_theta_1024x1024": prof["applies_by_rank"][10],
+            "energy_unnormalized": e, "norm": nrm, "energy": e / nrm if nrm > 0 else None,
+            "fidelity_est": st["fidelity_est"], "discarded_weight": st["discarded_weight"],
+            "max_bond": st["max_bond"], "jacobi_sweeps_max": st["jacobi_sweeps_max"]}
+
+
+def oracle_cost(job):
+    """Rough FP64 oracle cost of a brickwork circuit: 2-qubit gates x (its largest bond)^3."""
+    return C.count_2q(job["gates"]) * float(min(job["chi"], 2 ** (job["n"] // 2))) ** 3
 
 
 NOISY_N, NOISY_DEPTH = 40, 4
@@ -407,6 +498,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-noisy", action="store_true")
+    ap.add_argument("--no-qaoa60", action="store_true")
+    ap.add_argument("--oracle-sample", type=int, default=16,
+                    help="uneven-batch circuits (the cheapest) checked against the FP64 oracle")
     ap.add_argument("--uneven-circuits", type=int, default=256, help="circuits of the uneven batch")
     ap.add_argument("--serve-circuits", type=int, default=96, help="brickwork circuits in the serving stream")
     ap.add_argument("--serve-grid", type=int, default=1, help="6x6 grid circuits in the serving stream")
@@ -525,6 +619,7 @@ def main():
         out["speedup_vs_gate_at_a_time"] = round(value / (ga / gt), 3)
         log(f"  {ga} applies in {gt:.3f} s -> {ga / gt:.1f} applies/s")
     plain_own = None  # plain FP64 run time (s) of each uneven-batch circuit
+    oracle_sample = []  # uneven-batch results to check against the FP64 oracle in the cpu_baseline leg
     uneven = uneven_batch(args.uneven_circuits, 0x7E5A0004)
     if not args.no_uneven:
         jobs = uneven
@@ -554,6 +649,14 @@ def main():
                 "worst_circuit": {"qubits": jobs[worst]["n"], "chi_max": jobs[worst]["chi"],
                                   "fidelity_est": float(fid[worst]),
                                   "fp64_sensitivity": fp64_sensitivity(eng, jobs[worst], pez[worst])}}
+        # the cheapest circuits for the FP64 oracle, checked in the cpu_baseline leg against the oracle
+        order = sorted(range(len(jobs)), key=lambda k: oracle_cost(jobs[k]))[:args.oracle_sample]
+        for k in order:
+            oracle_sample.append({"job": jobs[k], "ez_mixed": bres[k]["expval_z"], "max_bond": bres[k]["max_bond"],
+                                  "fidelity_est": bres[k]["fidelity_est"],
+                                  "ez_fp64": pez[k] if not args.no_plain else None,
+                                  "fp64_sensitivity": fp64_sensitivity(eng, jobs[k], pez[k]) if not args.no_plain
+                                  else None})
         out["uneven_batch"] = ub
     if not args.no_serving:
         jobs = serving_stream(uneven[:args.serve_circuits], args.serve_grid, args.serve_grid_cycles, 0x7E5A0010)
@@ -599,6 +702,21 @@ def main():
                                                "circuit (one circuit at a time)"}
             serving["speedup_vs_plain"] = round(pspan / span, 3)
         out["serving"] = serving
+    if not args.no_qaoa60:
+        log("qaoa60: 60-qubit QAOA MaxCut p=4, swap network, chi_max 512, cutoff 1e-12")
+        q = {}
+        for prec in ("mixed", "fp64"):
+            q[prec] = qaoa_60(eng, prec)
+            log(f"  {prec}: {q[prec]['seconds']:.1f} s, {q[prec]['applies_per_s']:.1f} applies/s, energy "
+                f"{q[prec]['energy']} (unnormalised {q[prec]['energy_unnormalized']:.3e}), fidelity est "
+                f"{q[prec]['fidelity_est']:.2e}")
+        em, ef = q["mixed"]["energy"], q["fp64"]["energy"]
+        out["qaoa_60"] = {"workload": "60-qubit QAOA MaxCut, seeded random 3-regular graph (90 edges), p=4, gamma, "
+                                      "beta ~ U[0, pi], odd-even swap network (7080 2q applies), chi_max 512, "
+                                      "svd-cutoff 1e-12, one circuit alone",
+                          "mixed": q["mixed"], "fp64": q["fp64"],
+                          "energy_diff_mixed_vs_fp64": abs(em - ef) if em is not None and ef is not None else None,
+                          "speedup_mixed_vs_fp64": round(q["fp64"]["seconds"] / q["mixed"]["seconds"], 3)}
     noisy_ops, noisy_ez = None, None
     if not args.no_noisy:
         noisy_ops = noisy_workload()
@@ -616,8 +734,20 @@ def main():
                                     "max_abs_dz_across_copies": spread}
     eng.close()
     if not args.no_cpu_baseline:
-        log("cpu baseline (FP64 oracle)")
-        out["cpu_baseline"] = cpu_baseline()
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline: LAPACK zgesdd step on {threads} cores, then 1 core; Jacobi oracle")
+        out["cpu_baseline"] = cpu_baseline_lapack(threads)
+        out["cpu_baseline"]["cpu"] = cpu_info()
+        out["cpu_baseline"]["lapack_1_thread"] = cpu_baseline_lapack(1, seconds=6.0)
+        out["cpu_baseline"]["jacobi_oracle"] = cpu_baseline(seconds=6.0)
+        if oracle_sample:
+            log(f"cpu baseline: uneven-batch results against the FP64 oracle ({len(oracle_sample)} cheapest)")
+            checks, el = oracle_check_sample(oracle_sample, threads, budget_s=20.0)
+            zm = [c["z_error_mixed"] for c in checks]
+            out["cpu_baseline"]["uneven_oracle_check"] = {
+                "circuits": len(checks), "seconds": round(el, 2), "oracle": "oracle/mps_oracle.cpp FP64, same truncation",
+                "z_error_mixed_max": max(zm) if zm else None,
+                "bonds_equal": all(c["bond_equal"] for c in checks), "per_circuit": checks}
         if noisy_ops is not None:
             log("cpu baseline, noisy (purified-MPS oracle)")
             out["cpu_baseline"]["noisy"] = cpu_baseline_noisy(noisy_ops, noisy_ez)

@@ -163,6 +163,15 @@ int hipmps_amplitudes_noalloc(hipmps_state s, const int8_t* bits, int k, double*
 /* <psi| (x)_k diag(d[2k], d[2k+1]) |psi>: Z strings, projectors (sequential sampling,
  * getMeasureSample :2211-2364), occupation numbers */
 int hipmps_expval_diag(hipmps_state s, const double* diag_2n, double* out);
+/* getMeasureSample (ExaTnMpsVisitor.cpp:2211-2364, the reference's sampling for n >= 20) for `shots`
+ * shots at once, every shot's marginals advanced together on the device.  qubits: the k measured
+ * qubits in Measure order, which must be increasing (HIPMPS_E_UNSUPPORTED otherwise); draws: shots * k
+ * uniforms in [0, 1) in the order the reference takes them (shot by shot, one per measured qubit);
+ * out: shots * k chars '0' / '1', char i <-> qubits[i].  The first measured qubit is drawn against
+ * the probabilities of the unnormalised state, later ones against their conditionals; values below
+ * 1e-12 count as 0; outcome 0 when the draw is <= p0. */
+int hipmps_sample_sequential(hipmps_state s, const int* qubits, int k, int shots, const double* draws,
+                             char* out);
 /* deep copy of a state on the same engine (observable circuits in VQE mode) */
 int hipmps_state_clone(hipmps_state src, hipmps_state* out);
 int hipmps_state_vector(hipmps_state s, double* out_re_im, size_t cap); /* n <= 26 */

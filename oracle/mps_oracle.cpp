@@ -11,6 +11,7 @@
 //   * dense state vector ................... tnqvm/utils/GateMatrixAlgebra.hpp:63-123, 213-219
 //                                            (qubit k <-> bit k of the index)
 //   * sampling ............................. GateMatrixAlgebra.hpp:125-156 + utils/RandomEngine.hpp
+//   * sequential sampling (n >= 20) ........ ExaTnMpsVisitor.cpp:2211-2364 (getMeasureSample)
 //   * MPS initial state |0..0>, bonds 1 .... ExaTnMpsVisitor.cpp:280-326
 //   * 1q gate into a site .................. ExaTnMpsVisitor.cpp:1185-1292
 //   * 2q gate: merge, gate, full SVD with
@@ -633,6 +634,42 @@ double orc_mps_norm(void* h) {
   Mps* s = static_cast<Mps*>(h);
   std::vector<std::array<double, 2>> ops(s->n, {1.0, 1.0});
   return mps_diag_expect(*s, ops).real();
+}
+
+// getMeasureSample (ExaTnMpsVisitor.cpp:2211-2364), `shots` times in a row (the n >= 20 path of
+// finalize, :651-670), draws from one mt19937_64 seeded with `seed` through
+// uniform_real_distribution(0, 1) as randomEngine::randProb (RandomEngine.hpp:11-14), one draw
+// per measured qubit per shot in Measure order.  Per qubit: the diagonal of its reduced density
+// matrix with the earlier outcomes applied as collapse projectors scaled by 1/p of their outcome
+// (:2229-2276) -- the first qubit's probabilities are therefore those of the unnormalised state;
+// values below 1e-12 in magnitude count as 0 (:2329-2337); outcome 0 when the draw is <= p0
+// (:2342-2345).  out: shots * k chars, char i <-> meas[i].
+int orc_mps_measure_samples(void* h, uint64_t seed, int shots, const int* meas, int k, char* out) {
+  Mps* s = static_cast<Mps*>(h);
+  std::mt19937_64 eng;
+  eng.seed(seed);
+  constexpr double PROB_EPS = 1e-12;
+  for (int t = 0; t < shots; ++t) {
+    std::vector<std::array<double, 2>> ops(s->n, {1.0, 1.0});
+    for (int i = 0; i < k; ++i) {
+      const int q = meas[i];
+      const std::array<double, 2> keep = ops[q];
+      double prob[2];
+      for (int b = 0; b < 2; ++b) {
+        ops[q] = {b == 0 ? keep[0] : 0.0, b == 1 ? keep[1] : 0.0};
+        const double v = mps_diag_expect(*s, ops).real();
+        prob[b] = std::fabs(v) < PROB_EPS ? 0.0 : v;
+      }
+      const double r = std::uniform_real_distribution<double>(0.0, 1.0)(eng);
+      const int bit = r <= prob[0] ? 0 : 1;
+      const double p = prob[bit];
+      // collapse onto the outcome, the ket scaled by 1/p (the bra is not projected: for the
+      // diagonal entries used here one projector gives the same value)
+      ops[q] = {bit == 0 ? keep[0] / p : 0.0, bit == 1 ? keep[1] / p : 0.0};
+      out[(size_t)t * k + i] = bit ? '1' : '0';
+    }
+  }
+  return shots;
 }
 
 // amplitudes of k full bit strings (bits: k*n int8, 0/1)

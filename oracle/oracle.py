@@ -69,6 +69,8 @@ def lib():
         L.orc_mps_amplitudes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int8),
                                          ctypes.c_int, _dp]
         L.orc_mps_slice.argtypes = [ctypes.c_void_p, _ip, _dp]
+        L.orc_mps_measure_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, _ip, ctypes.c_int,
+                                              ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -240,3 +242,11 @@ class Mps:
         out = np.zeros(1 << nopen, dtype=np.complex128)
         lib().orc_mps_slice(self.h, _i(b), _d(out.view(np.float64)))
         return out
+
+    def measure_samples(self, seed: int, shots: int, meas) -> list:
+        """getMeasureSample `shots` times (the reference's n >= 20 sampling), mt19937_64(seed)."""
+        m = np.ascontiguousarray(np.asarray(meas, dtype=np.int32))
+        buf = ctypes.create_string_buffer(shots * len(m))
+        lib().orc_mps_measure_samples(self.h, seed, shots, _i(m), len(m), buf)
+        raw = buf.raw
+        return [raw[i * len(m):(i + 1) * len(m)].decode() for i in range(shots)]

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -16,6 +17,20 @@
 #include "TNQVM.hpp"
 
 using namespace xacc;
+
+// The CPU oracle (oracle/mps_oracle.cpp, the checker) for the sampling parity cases
+extern "C" {
+int orc_gate_matrix(const char* name, const double* params, int nparams, double* out);
+void orc_sv_init(int n, double* psi);
+void orc_sv_apply_1q(int n, double* psi, int q, const double* m);
+void orc_sv_apply_2q(int n, double* psi, int q0, int q1, const double* m);
+int orc_sv_sample(int n, const double* psi, uint64_t seed, int shots, const int* meas, int k, char* out);
+void* orc_mps_create(int n, double cutoff, int max_bond, int pn_l2, int nthreads);
+void orc_mps_destroy(void* h);
+int orc_mps_apply_1q(void* h, int q, const double* m);
+int orc_mps_apply_2q(void* h, int q0, int q1, const double* m);
+int orc_mps_measure_samples(void* h, uint64_t seed, int shots, const int* meas, int k, char* out);
+}
 
 namespace {
 int g_fail = 0, g_checks = 0;
@@ -288,6 +303,90 @@ CASE(measure_ghz35, true) {
       for (int i = 0; i < 34; i++) { CNOT(q[i], q[i + 1]); }
       Measure(q[5]); Measure(q[3]); Measure(q[7]); Measure(q[34]); })"));
   EXPECT_NEAR(q->computeMeasurementProbability("0000") + q->computeMeasurementProbability("1111"), 1.0, 1e-12);
+}
+
+// ---- sampling parity with the oracle under the same seed ------------------------------------
+// A seeded line circuit of U(theta, phi, lambda) layers and CNOT brickwork as XASM
+std::string line_circuit(int n, int depth, uint64_t seed, const std::vector<int>& meas) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> ang(0.0, 2.0 * constants::pi);
+  std::string src = "__qpu__ void c(qbit q) {\n";
+  char buf[160];
+  for (int l = 0; l < depth; ++l) {
+    for (int q = 0; q < n; ++q) {
+      std::snprintf(buf, sizeof(buf), "  U(q[%d], %.17g, %.17g, %.17g);\n", q, ang(rng), ang(rng), ang(rng));
+      src += buf;
+    }
+    for (int q = l % 2; q + 1 < n; q += 2) {
+      std::snprintf(buf, sizeof(buf), "  CNOT(q[%d], q[%d]);\n", q, q + 1);
+      src += buf;
+    }
+  }
+  for (int q : meas) {
+    std::snprintf(buf, sizeof(buf), "  Measure(q[%d]);\n", q);
+    src += buf;
+  }
+  return src + "}";
+}
+
+// gates of a composite through the oracle's matrices: apply1(q, m), apply2(q0, q1, m)
+template <class F1, class F2>
+void run_on_oracle(const std::shared_ptr<CompositeInstruction>& f, F1 apply1, F2 apply2) {
+  for (auto& g : f->getInstructions()) {
+    if (g->name() == "Measure" || g->name() == "I") continue;
+    std::vector<double> p;
+    for (auto& x : g->getParameters()) p.push_back(x.as<double>());
+    double m[32];
+    const int nq = orc_gate_matrix(g->name().c_str(), p.data(), (int)p.size(), m);
+    if (nq == 1) apply1((int)g->bits()[0], m);
+    else apply2((int)g->bits()[0], (int)g->bits()[1], m);
+  }
+}
+
+std::map<std::string, int> count(const std::vector<char>& raw, int shots, int k) {
+  std::map<std::string, int> c;
+  for (int t = 0; t < shots; ++t) c[std::string(raw.data() + (size_t)t * k, (size_t)k)]++;
+  return c;
+}
+
+// n < 20: GenerateSamples over the state vector (GateMatrixAlgebra.hpp:125-156)
+CASE(sample_parity_state_vector, true) {
+  const int n = 10, shots = 3000;
+  const std::vector<int> meas = {7, 2, 9, 0, 4};
+  auto f = compile1(line_circuit(n, 6, 11, meas));
+  auto acc = mps({{"shots", shots}, {"seed", 2024}});
+  auto q = qalloc(n);
+  acc->execute(q, f);
+  std::vector<double> psi(2 * ((size_t)1 << n));
+  orc_sv_init(n, psi.data());
+  run_on_oracle(f, [&](int a, const double* m) { orc_sv_apply_1q(n, psi.data(), a, m); },
+                [&](int a, int b, const double* m) { orc_sv_apply_2q(n, psi.data(), a, b, m); });
+  std::vector<char> raw((size_t)shots * meas.size());
+  EXPECT_EQ(orc_sv_sample(n, psi.data(), 2024, shots, meas.data(), (int)meas.size(), raw.data()), shots);
+  EXPECT_TRUE(q->getMeasurementCounts() == count(raw, shots, (int)meas.size()));
+}
+
+// n >= 20: getMeasureSample (ExaTnMpsVisitor.cpp:2211-2364), measured qubits in increasing order
+// (every shot advanced together on the device) and in a scrambled order (shot by shot), untruncated
+// and with max-bond-dim 8 (norm below 1)
+CASE(sample_parity_sequential, true) {
+  const int n = 22, shots = 150;
+  const std::vector<std::vector<int>> orders = {{0, 3, 4, 9, 15, 21}, {15, 3, 21, 0}};
+  for (int cap : {0, 8}) {
+    for (const auto& meas : orders) {
+      auto f = compile1(line_circuit(n, 5, 12, meas));
+      auto acc = cap ? mps({{"shots", shots}, {"seed", 77}, {"max-bond-dim", cap}}) : mps({{"shots", shots}, {"seed", 77}});
+      auto q = qalloc(n);
+      acc->execute(q, f);
+      void* o = orc_mps_create(n, 2.2250738585072014e-308, cap, 0, 1);
+      run_on_oracle(f, [&](int a, const double* m) { orc_mps_apply_1q(o, a, m); },
+                    [&](int a, int b, const double* m) { orc_mps_apply_2q(o, a, b, m); });
+      std::vector<char> raw((size_t)shots * meas.size());
+      orc_mps_measure_samples(o, 77, shots, meas.data(), (int)meas.size(), raw.data());
+      orc_mps_destroy(o);
+      EXPECT_TRUE(q->getMeasurementCounts() == count(raw, shots, (int)meas.size()));
+    }
+  }
 }
 
 // MpsMeasurementTester.cpp:37-66

@@ -114,3 +114,22 @@ def test_overflow_reported(eng):
         s.flush()
     assert ei.value.code == capi.HIPMPS_E_NUMERIC
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_rank_deficient_theta_converges(eng, precision):
+    """The 60-qubit QAOA swap network produces exactly rank-deficient theta' (here 50 x 56 of rank
+    28) whose remaining rows are rounding residue (norms 1e-16 ... 1e-93).  A purely relative
+    orthogonality test keeps rotating such rows against the big ones for ever; rows below
+    1e-15 ||X||_F count as zero, and every SVD converges."""
+    from tnqvm_amd import circuits as C
+    gates, _, _, _ = C.qaoa_maxcut(60, 4, seed=0x7E5A0003)
+    s = mps.MpsState(eng, 60, cutoff=1e-12, max_bond=512, precision=precision)
+    for k in range(0, 660, 30):
+        s.run(gates[k:k + 30])
+        s.flush()
+    st = s.stats()
+    assert st["svds_nonconverged"] == 0
+    assert st["max_bond"] == 512
+    assert np.all(np.isfinite(s.expval_z_all()))
+    s.close()

@@ -16,6 +16,7 @@ import os
 import numpy as np
 import pytest
 
+from mt64 import Mt64 as _Mt64
 from oracle import oracle as O
 from tnqvm_amd import circuits as C
 
@@ -225,3 +226,91 @@ def test_truncation_discards_smallest():
     m = O.Mps(2, max_bond=1).run([gate])
     assert abs(m.norm() - s[0] ** 2) < 1e-13
     assert abs(m.discarded() - s[1] ** 2) < 1e-13
+
+
+def _rand_adjacent(n, n_gates, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_gates):
+        if rng.random() < 0.5:
+            q = int(rng.integers(0, n - 1))
+            pair = (q, q + 1) if rng.random() < 0.5 else (q + 1, q)
+            out.append({"name": "unitary", "qubits": pair, "matrix": C.haar_unitary(rng, 4)})
+        else:
+            out.append({"name": "unitary", "qubits": (int(rng.integers(0, n)),), "matrix": C.haar_unitary(rng, 2)})
+    return out
+
+
+@pytest.mark.parametrize("cutoff,max_bond,pn_l2", [(0.0, 0, False), (1e-3, 0, False), (1e-8, 4, False),
+                                                   (1e-2, 0, True)])
+def test_lapack_mps_matches_oracle(cutoff, max_bond, pn_l2):
+    """The zgesdd restatement (the LAPACK-class CPU baseline) against the Jacobi oracle on the same
+    truncated circuits: bonds, state and discarded weight; without truncation, also the dense vector."""
+    from oracle.lapack_mps import LapackMps
+    for seed in range(3):
+        n = 7
+        gates = _rand_adjacent(n, 50, seed=40 + seed)
+        ref = O.Mps(n, cutoff=cutoff, max_bond=max_bond, pn_l2=pn_l2).run(gates)
+        lap = LapackMps(n, cutoff=cutoff, max_bond=max_bond, pn_l2=pn_l2).run(gates, O.gate_of)
+        assert lap.bonds() == list(ref.bonds())
+        assert np.abs(lap.state_vector() - ref.state_vector()).max() < 1e-10
+        assert abs(lap.discarded - ref.discarded()) < 1e-10
+        if cutoff == 0.0 and max_bond == 0:
+            assert np.abs(lap.state_vector() - O.StateVector(n).run(gates).psi).max() < 1e-12
+
+
+def test_mt64_standard_value():
+    # [rand.predef]: the 10000th consecutive output of a default-constructed mt19937_64
+    g = _Mt64()
+    for _ in range(9999):
+        g.next()
+    assert g.next() == 9981545732273789042
+
+
+def test_measure_samples_product_state_closed_form():
+    """getMeasureSample on a product state Ry(t_q)|0> whose first site is scaled by c (norm c^2):
+    the first measured qubit is drawn against its unnormalised probability c^2 cos^2(t/2), every
+    later one against its conditional cos^2(t/2); one mt19937_64 draw per qubit per shot."""
+    n, shots, seed, c = 5, 200, 99, 0.8
+    th = [0.3, 1.1, 2.0, 2.9, 0.7]
+    m = O.Mps(n).run([{"name": "Ry", "qubits": (q,), "params": (t,)} for q, t in enumerate(th)])
+    a0 = m.site(0) * c
+    m.set_site(0, a0)
+    meas = [3, 0, 4, 1]
+    got = m.measure_samples(seed, shots, meas)
+    g = _Mt64(seed)
+    p0 = {q: np.cos(th[q] / 2) ** 2 for q in range(n)}
+    want = []
+    for _ in range(shots):
+        s = ""
+        for i, q in enumerate(meas):
+            p = p0[q] * (c * c if i == 0 else 1.0)  # the first measured qubit sees the norm c^2
+            s += "0" if g.uniform() <= p else "1"
+        want.append(s)
+    assert got == want
+
+
+def test_measure_samples_ghz_and_dense_statistics():
+    """GHZ: the first draw decides every bit; a random 8-qubit state: the counts of 4000 shots agree
+    with the dense distribution of the measured qubits (chi-square below a loose bound)."""
+    n = 6
+    m = O.Mps(n).run([{"name": "H", "qubits": (0,)}] + [{"name": "CNOT", "qubits": (i, i + 1)} for i in range(n - 1)])
+    got = m.measure_samples(5, 64, [2, 0, 5])
+    g = _Mt64(5)
+    want = []
+    for _ in range(64):
+        r = [g.uniform() for _ in range(3)]
+        want.append("000" if r[0] <= 0.5 else "111")
+    assert got == want
+    gates = _rand_adjacent(8, 60, seed=77)
+    sv = O.StateVector(8).run(gates)
+    mm = O.Mps(8).run(gates)
+    meas = [1, 4, 6]
+    got = mm.measure_samples(11, 4000, meas)
+    p = np.abs(sv.psi) ** 2
+    probs = {}
+    for x in range(256):
+        key = "".join("1" if (x >> q) & 1 else "0" for q in meas)
+        probs[key] = probs.get(key, 0.0) + p[x]
+    chi2 = sum((got.count(k) - 4000 * v) ** 2 / (4000 * v) for k, v in probs.items() if v > 1e-9)
+    assert chi2 < 40.0  # 7 degrees of freedom

@@ -38,7 +38,7 @@ EXPORTS = [
     "hipmps_expval_diag", "hipmps_state_clone", "hipmps_apply_kraus", "hipmps_kraus_dims",
     "hipmps_reduced_density_matrix", "hipmps_engine_get_tuning", "hipmps_engine_set_tuning",
     "hipmps_queue_gates", "hipmps_queue_kraus", "hipmps_run_waves", "hipmps_circuit_footprint",
-    "hipmps_amplitudes_noalloc",
+    "hipmps_amplitudes_noalloc", "hipmps_sample_sequential",
 ]
 
 HIPMPS_NOALLOC_SCRATCH_BYTES = 16 << 20
@@ -137,6 +137,7 @@ def lib():
         "hipmps_expval_zz": ([vp, pint, ip, dp], ip),
         "hipmps_amplitudes": ([vp, ctypes.POINTER(ctypes.c_int8), ip, dp], ip),
         "hipmps_amplitudes_noalloc": ([vp, ctypes.POINTER(ctypes.c_int8), ip, dp], ip),
+        "hipmps_sample_sequential": ([vp, pint, ip, ip, dp, ctypes.c_char_p], ip),
         "hipmps_wavefunction_slice": ([vp, pint, dp, ctypes.c_size_t], ip),
         "hipmps_state_vector": ([vp, dp, ctypes.c_size_t], ip),
         "hipmps_get_site": ([vp, ip, dp, ctypes.c_size_t], ip),

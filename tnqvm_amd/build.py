@@ -70,9 +70,12 @@ def build_visitor() -> str:
 
 
 def build_cpp_tests() -> list:
+    """The C++ visitor tests; they link the CPU oracle (oracle/liboracle.so) as their checker."""
     tdir = os.path.join(ROOT, "tests", "cpp")
     if not os.path.isdir(tdir):
         return []
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir])
     bdir = os.path.join(tdir, "bin")
     os.makedirs(bdir, exist_ok=True)
     out = []
@@ -81,8 +84,8 @@ def build_cpp_tests() -> list:
             continue
         exe = os.path.join(bdir, os.path.splitext(f)[0])
         _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + HOST, "-o", exe,
-              os.path.join(tdir, f), "-L" + PKG, "-ltnqvm_hipmps", "-lhipmps",
-              "-Wl,-rpath," + PKG])
+              os.path.join(tdir, f), "-L" + PKG, "-ltnqvm_hipmps", "-lhipmps", "-L" + odir, "-loracle",
+              "-Wl,-rpath," + PKG, "-Wl,-rpath," + odir])
         out.append(exe)
     return out
 

@@ -569,12 +569,13 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   }
   RC_TRY(E->ws.ensure(off));
   char* ws = (char*)E->ws.p;
-  // small: counts (2N ints), K (N ints), wstats (WS_N doubles per item)
-  size_t small_bytes = align_up(3 * N * sizeof(int)) + WS_N * N * sizeof(double);
+  // small: counts (2N ints), K (N ints), wstats (WS_N doubles per item), ||X||_F^2 (N doubles)
+  size_t small_bytes = align_up(3 * N * sizeof(int)) + align_up(WS_N * N * sizeof(double)) + N * sizeof(double);
   RC_TRY(E->small.ensure(small_bytes));
   int* d_counts = (int*)E->small.p;
   int* d_K = d_counts + 2 * N;
   double* d_w = (double*)((char*)E->small.p + align_up(3 * N * sizeof(int)));
+  double* d_fro2 = (double*)((char*)d_w + align_up(WS_N * N * sizeof(double)));
   // descriptors
   size_t desc_bytes = align_up(N * sizeof(ThetaTask)) + align_up(N * sizeof(SplitTask)) +
                       align_up(N * sizeof(JacTask)) + align_up(N * sizeof(GemmTask)) +
@@ -678,6 +679,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     J.nb = it.r_pad / JB;
     J.counter = d_counts + 2 * q;
     J.visited = d_counts + 2 * q + 1;
+    J.fro2 = d_fro2 + q;
     J.tol = it.fp32 ? E->tol_mixed : E->tol;
     J.gram32 = 0;
     J.pad = 0;
@@ -778,6 +780,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   if (n64 > 0) HIP_TRY(launch_theta(dth, n64, tile_acc[0], 0, st));
   if (N - n64 > 0) HIP_TRY(launch_theta(dth + n64, N - n64, tile_acc[1], 1, st));
   HIP_TRY(launch_jacobi_prep(dsp, djt, N, max_rpad, max_c, st));
+  HIP_TRY(launch_fro2(dsp, djt, N, st));
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[6], st));
   // LQ lanes: with many items, contiguous slices of the LQ task arrays run their panel
   // sequences on separate streams, so that one slice's launch tails overlap another's work
@@ -903,6 +906,12 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       for (int a = 0; a < nact; ++a) R += hcounts[2 * active[a]], V += hcounts[2 * active[a] + 1];
       fprintf(stderr, "[hipmps sweep] wave %lld sweep %d active %d rotated %lld visited %lld\n", E->prof.waves, sweep,
               nact, R, V);
+      if (sweep >= 25)
+        for (int a = 0; a < nact; ++a) {
+          const int q = active[a];
+          fprintf(stderr, "[hipmps sweep]   item %d: m %d n %d r %d lq %d fp32 %d rotated %d visited %d\n", q,
+                  items[q].m, items[q].n, items[q].r, items[q].lq, items[q].fp32, hcounts[2 * q], hcounts[2 * q + 1]);
+        }
     }
     std::vector<int> next;
     for (int a = 0; a < nact; ++a) {
@@ -1234,7 +1243,10 @@ int run_kraus_waves(hipmps_engine E, const std::vector<hipmps_state_s*>& states)
     J.nb = h.r_pad / JB;
     J.counter = d_counts + 2 * i;
     J.visited = d_counts + 2 * i + 1;
+    J.fro2 = nullptr;
     J.tol = s->fp32 ? E->tol_mixed : E->tol;
+    J.gram32 = 0;
+    J.pad = 0;
     ko += 4 * (size_t)h.n_ops;
   }
   char* dd = (char*)E->desc.p;
@@ -1722,6 +1734,94 @@ int measure_diag(hipmps_state_s* s, const std::vector<double>& diag, double* out
   double2 v;
   RC_TRY(read_scalar(E, (char*)E->env.p + L.offL[s->n], &v));
   *out = v.x;
+  return 0;
+}
+
+// getMeasureSample (ExaTnMpsVisitor.cpp:2211-2364) for many shots at once, the measured qubits in
+// increasing site order.  Every shot carries its own left environment E_t across the chain
+// (the traced sites contract ket with bra, a measured site keeps only the shot's outcome, scaled
+// by 1/p as the reference's collapse tensors are); at a measured site one launch forms both
+// outcome branches A_b^H E_t A_b of every shot, one trace against the shared right environment
+// gives their probabilities, and the host applies the shot's draw (outcome 0 when the draw is <=
+// p0, values below 1e-12 counted as 0).  The first measured qubit therefore sees the
+// probabilities of the unnormalised state, every later one its conditional probabilities, as in
+// the reference.  Shots run in blocks whose environments fit a bounded workspace.
+int sample_measure(hipmps_state_s* s, const int* qubits, int k, int shots, const double* draws, char* out) {
+  hipmps_engine E = s->e;
+  const int n = s->n;
+  int cmax = 1;
+  for (int b = 0; b <= n; ++b) cmax = std::max(cmax, s->chi[b]);
+  const EnvLayout L = env_layout(s, 1, 0);
+  const size_t mat = align_up((size_t)cmax * cmax * 16);
+  const size_t xs = align_up((size_t)cmax * 2 * max_kap(s) * cmax * 16);
+  const size_t per_shot = 3 * mat + 2 * xs;  // E and two candidates, two X scratches
+  const size_t budget = std::max<size_t>((size_t)256 << 20, 8 * per_shot);
+  const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)shots, budget / per_shot));
+  const size_t tables = 2 * align_up(2 * (size_t)B * sizeof(void*)) + align_up(2 * (size_t)B * sizeof(int)) +
+                        2 * (size_t)B * sizeof(double2) + 256;
+  RC_TRY(E->env.ensure(L.total + (size_t)B * per_shot + tables));
+  char* base = (char*)E->env.p;
+  RC_TRY(compute_right_envs(s, L));
+  char* sb = base + L.total;
+  char* tb = sb + (size_t)B * per_shot;
+  auto mat_at = [&](int t, int i) { return (double2*)(sb + (size_t)t * per_shot + (size_t)i * mat); };
+  auto x_at = [&](int t, int b) { return (double2*)(sb + (size_t)t * per_shot + 3 * mat + (size_t)b * xs); };
+  constexpr double PROB_EPS = 1e-12;
+  for (int s0 = 0; s0 < shots; s0 += B) {
+    const int nb = std::min(B, shots - s0);
+    std::vector<int> cur(nb, 0);
+    std::vector<double> f(nb, 1.0);  // product of 1/p of the outcomes so far
+    {  // E_t = [1] (bond 0 has dimension 1); the stream is idle here (every step above ended synchronised)
+      const std::vector<double2> ones(nb, make_double2(1.0, 0.0));
+      HIP_TRY(hipMemcpy2D(mat_at(0, 0), per_shot, ones.data(), sizeof(double2), sizeof(double2), nb,
+                          hipMemcpyHostToDevice));
+    }
+    int mi = 0;
+    for (int site = 0; site < n && mi < k; ++site) {
+      const bool meas = qubits[mi] == site;
+      std::vector<EnvJob> jobs;
+      jobs.reserve((size_t)nb * 2);
+      for (int t = 0; t < nb; ++t) {
+        for (int b = 0; b < (meas ? 2 : 1); ++b) {
+          EnvJob j{s->sites[site].p, s->chi[site], s->chi[site + 1], mat_at(t, cur[t]), x_at(t, b),
+                   mat_at(t, (cur[t] + 1 + b) % 3), meas ? 1 : 0};
+          j.kap = s->kap[site];
+          if (meas) {
+            j.d0 = b == 0 ? 1.0 : 0.0;
+            j.d1 = b == 1 ? 1.0 : 0.0;
+          }
+          jobs.push_back(j);
+        }
+      }
+      RC_TRY(env_steps(E, jobs, true, s->fp32));
+      if (!meas) {
+        for (int t = 0; t < nb; ++t) cur[t] = (cur[t] + 1) % 3;
+        continue;
+      }
+      std::vector<const double2*> A, R;
+      std::vector<int> dims;
+      for (int t = 0; t < nb; ++t)
+        for (int b = 0; b < 2; ++b) {
+          A.push_back(mat_at(t, (cur[t] + 1 + b) % 3));
+          R.push_back((const double2*)(base + L.offR[site + 1]));
+          dims.push_back(s->chi[site + 1]);
+        }
+      std::vector<double2> pr;
+      RC_TRY(trace_products(E, A, R, dims, tb, pr));
+      for (int t = 0; t < nb; ++t) {
+        double prob[2];
+        for (int b = 0; b < 2; ++b) {
+          prob[b] = pr[2 * t + b].x * f[t];
+          if (std::fabs(prob[b]) < PROB_EPS) prob[b] = 0.0;
+        }
+        const int bit = draws[(size_t)(s0 + t) * k + mi] <= prob[0] ? 0 : 1;
+        out[(size_t)(s0 + t) * k + mi] = bit ? '1' : '0';
+        if (prob[bit] > 0.0) f[t] /= prob[bit];
+        cur[t] = (cur[t] + 1 + bit) % 3;
+      }
+      ++mi;
+    }
+  }
   return 0;
 }
 
@@ -2466,6 +2566,18 @@ int hipmps_expval_diag(hipmps_state s, const double* diag, double* out) {
   RC_TRY(state_flush(s));
   std::vector<double> d(diag, diag + 2 * s->n);
   return measure_diag(s, d, out);
+}
+
+int hipmps_sample_sequential(hipmps_state s, const int* qubits, int k, int shots, const double* draws, char* out) {
+  if (!s || !qubits || k < 1 || shots < 0 || (shots > 0 && (!draws || !out))) return fail(HIPMPS_E_ARG, "bad arguments");
+  for (int i = 0; i < k; ++i) {
+    if (qubits[i] < 0 || qubits[i] >= s->n) return fail(HIPMPS_E_ARG, "measured qubit out of range");
+    if (i > 0 && qubits[i] <= qubits[i - 1])
+      return fail(HIPMPS_E_UNSUPPORTED, "measured qubits must be in increasing order");
+  }
+  if (shots == 0) return 0;
+  RC_TRY(state_flush(s));
+  return sample_measure(s, qubits, k, shots, draws, out);
 }
 
 int hipmps_state_clone(hipmps_state src, hipmps_state* out) {

@@ -39,6 +39,9 @@ struct JacTask {
   int pair_begin;    // first block of this task in the grid
   int* counter;      // rotated block pairs in the current sweep
   int* visited;      // block pairs visited in the current sweep
+  double* fro2;      // ||X||_F^2 (k_fro2), or null: rows with squared norm below JAC_FLOOR2 * fro2 are
+                     // numerically zero (rounding residue) and neither rotate nor count against
+                     // convergence
   double tol;        // relative off-diagonal below which a block pair counts as orthogonal
   int gram32;        // this sweep forms the Gram with FP32 MFMA (rotations and updates stay FP64)
   int pad;
@@ -141,6 +144,11 @@ hipError_t launch_theta(const ThetaTask* d_tasks, int ntasks, int total_tiles, i
                         hipStream_t s);
 hipError_t launch_jacobi_prep(const SplitTask* d_split, const JacTask* d_jac, int ntasks,
                               int max_rows, int max_cols, hipStream_t s);
+// (1e-15)^2: a row whose norm is below 1e-15 ||X||_F lies under FP64 resolution of any result
+// computed from X, so the relative convergence test has no meaning for it
+constexpr double JAC_FLOOR2 = 1e-30;
+// *jt[t].fro2 = ||theta'_t||_F^2 (= ||X||_F^2 = ||R2||_F^2), one workgroup per task, fixed order
+hipError_t launch_fro2(const SplitTask* d_split, const JacTask* d_jac, int ntasks, hipStream_t s);
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
                                int inner_sweeps, hipStream_t s);
 // Householder LQ of every task's M (panel by panel, one launch per panel).

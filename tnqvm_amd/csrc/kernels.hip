@@ -252,10 +252,10 @@ __device__ __forceinline__ double nr_rcp(double d) {
 // tau = 2 sign(h) gamma / (|h| + sqrt(h^2 + 4|gamma|^2))  (|tau| = t, the smaller root),
 // c = 1/sqrt(1 + |tau|^2), se = c tau;  J = [[c, se], [-conj(se), c]].
 // Returns the identity when |gamma| <= tol sqrt(alpha beta) or a diagonal entry is 0.
-__device__ __forceinline__ void jacobi_rotation(double alpha, double beta, double2 gamma, double tol, double& c,
-                                                double2& se) {
+__device__ __forceinline__ void jacobi_rotation(double alpha, double beta, double2 gamma, double tol, double floor2,
+                                                double& c, double2& se) {
   const double g2 = cabs2(gamma);
-  if (!(alpha > 0.0) || !(beta > 0.0) || !(g2 > tol * tol * alpha * beta)) {
+  if (!(alpha > floor2) || !(beta > floor2) || !(g2 > tol * tol * alpha * beta)) {
     c = 1.0;
     se = make_double2(0.0, 0.0);
     return;
@@ -306,6 +306,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   const int C = T.c, CT = T.c_tot;
   const size_t ld = T.ldy;
   const double tol = T.tol;
+  const double floor2 = T.fro2 ? JAC_FLOOR2 * *T.fro2 : 0.0;
   double2* Y = T.Y;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
       int i = e / JP, j = e % JP;
       if (i == j) continue;
       double aii = sG[i][i].x, ajj = sG[j][j].x;
-      if (aii > 0.0 && ajj > 0.0) m = fmax(m, cabs2(sG[i][j]) / (aii * ajj));
+      if (aii > floor2 && ajj > floor2) m = fmax(m, cabs2(sG[i][j]) / (aii * ajj));
     }
     s_red[tid] = m;
     __syncthreads();
@@ -469,7 +470,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     const int p = circle_slot(tid, 0, JP), q = circle_slot(JP - 1 - tid, 0, JP);
     double c;
     double2 se;
-    jacobi_rotation(sG[p][p].x, sG[q][q].x, sG[p][q], itol, c, se);
+    jacobi_rotation(sG[p][p].x, sG[q][q].x, sG[p][q], itol, floor2, c, se);
     s_c[0][tid] = c;
     s_se[0][tid] = se;
   }
@@ -523,7 +524,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     const double2 gam = rra ? (rrb ? n11 : n10) : (rrb ? n01 : n00);
     double cn;
     double2 sen;
-    jacobi_rotation(alpha, beta, gam, itol, cn, sen);
+    jacobi_rotation(alpha, beta, gam, itol, floor2, cn, sen);
     // ---- all LDS writes of the step ----
     gB[pk][pl] = n00;
     gB[pk][ql] = n01;
@@ -580,6 +581,27 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     }
   }
   if (tid == 0) atomicAdd(T.counter, 1);
+}
+
+__global__ __launch_bounds__(256) void k_fro2(const SplitTask* sp, const JacTask* jt) {
+  __shared__ double red[256];
+  const SplitTask& S = sp[blockIdx.x];
+  const size_t n = (size_t)S.m * S.n;
+  double a = 0.0;
+  for (size_t e = threadIdx.x; e < n; e += 256) a += cabs2(S.theta[e]);
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && jt[blockIdx.x].fro2) *jt[blockIdx.x].fro2 = red[0];
+}
+
+hipError_t launch_fro2(const SplitTask* d_split, const JacTask* d_jac, int ntasks, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fro2, dim3(ntasks), dim3(256), 0, s, d_split, d_jac);
+  return hipGetLastError();
 }
 
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,

@@ -150,34 +150,45 @@ std::vector<std::string> HipMpsVisitor::sample_state_vector(int shots) {
   return out;
 }
 
-// getMeasureSample (ExaTnMpsVisitor.cpp:2211-2364): qubit by qubit in Measure order,
-// the marginal of the next qubit given the outcomes drawn so far; outcome 0 when the
-// draw is <= p0; probabilities below 1e-12 count as 0.
+// getMeasureSample (ExaTnMpsVisitor.cpp:2211-2364): qubit by qubit in Measure order, the
+// diagonal of the qubit's reduced density matrix with the earlier outcomes applied as collapse
+// projectors scaled by 1/p of their outcome (:2229-2276), so the first qubit sees the
+// probabilities of the unnormalised state and every later one its conditional probabilities;
+// values below 1e-12 in magnitude count as 0 (:2329-2337); outcome 0 when the draw is <= p0
+// (:2342-2345).  One draw per measured qubit per shot, shots in a row.
 std::vector<std::string> HipMpsVisitor::sample_sequential(int shots) {
   std::vector<std::string> out;
+  const int k = (int)m_measured.size();
+  bool increasing = true;
+  for (int i = 1; i < k; ++i) increasing = increasing && m_measured[i] > m_measured[i - 1];
+  if (increasing && k > 0) {  // every shot's marginals advanced together on the device
+    std::vector<double> draws((size_t)shots * k);
+    for (auto& r : draws) r = randomEngine::get_instance().randProb();
+    std::vector<int> q(m_measured.begin(), m_measured.end());
+    std::vector<char> bits((size_t)shots * k + 1);
+    check(hipmps_sample_sequential(m_state, q.data(), k, shots, draws.data(), bits.data()), "sampling");
+    for (int s = 0; s < shots; ++s) out.emplace_back(bits.data() + (size_t)s * k, (size_t)k);
+    return out;
+  }
   std::vector<double> diag(2 * m_n);
   for (int s = 0; s < shots; ++s) {
     std::fill(diag.begin(), diag.end(), 1.0);
     std::string bits;
     for (int q : m_measured) {
-      double w[2];
+      const double k0 = diag[2 * q], k1 = diag[2 * q + 1];
+      double prob[2];
       for (int b = 0; b < 2; ++b) {
-        const double k0 = diag[2 * q], k1 = diag[2 * q + 1];
         diag[2 * q] = b == 0 ? k0 : 0.0;
         diag[2 * q + 1] = b == 1 ? k1 : 0.0;
-        check(hipmps_expval_diag(m_state, diag.data(), &w[b]), "marginal");
-        diag[2 * q] = k0;
-        diag[2 * q + 1] = k1;
+        check(hipmps_expval_diag(m_state, diag.data(), &prob[b]), "marginal");
+        if (std::fabs(prob[b]) < 1e-12) prob[b] = 0.0;
       }
-      const double tot = w[0] + w[1];
-      double p0 = tot > 0 ? w[0] / tot : 0.5;
-      if (std::fabs(p0) < 1e-12) p0 = 0.0;
-      if (std::fabs(1.0 - p0) < 1e-12) p0 = 1.0;
-      const int b = randomEngine::get_instance().randProb() <= p0 ? 0 : 1;
+      const int b = randomEngine::get_instance().randProb() <= prob[0] ? 0 : 1;
       bits.push_back(b ? '1' : '0');
-      // collapse: keep only outcome b on this qubit for the following marginals
-      diag[2 * q] *= (b == 0);
-      diag[2 * q + 1] *= (b == 1);
+      // collapse onto outcome b, scaled by 1/p of that outcome
+      const double sc = prob[b] > 0.0 ? 1.0 / prob[b] : 1.0;
+      diag[2 * q] = b == 0 ? k0 * sc : 0.0;
+      diag[2 * q + 1] = b == 1 ? k1 * sc : 0.0;
     }
     out.push_back(bits);
   }

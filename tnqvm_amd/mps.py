@@ -303,6 +303,18 @@ class MpsState:
                                               out.view(np.float64).ctypes.data_as(_dp)))
         return out
 
+    def sample_sequential(self, qubits, draws) -> list:
+        """getMeasureSample for len(draws) // len(qubits) shots with the given uniforms (shot-major,
+        one per measured qubit); qubits in increasing order.  Returns the strings."""
+        q = np.ascontiguousarray(np.asarray(qubits, dtype=np.int32))
+        d = np.ascontiguousarray(np.asarray(draws, dtype=np.float64).reshape(-1))
+        k = len(q)
+        shots = d.size // k
+        buf = ctypes.create_string_buffer(max(1, shots * k))
+        check(lib().hipmps_sample_sequential(self.h, q.ctypes.data_as(_ip), k, shots, d.ctypes.data_as(_dp), buf))
+        raw = buf.raw
+        return [raw[i * k:(i + 1) * k].decode() for i in range(shots)]
+
     def slice(self, bits) -> np.ndarray:
         b = np.ascontiguousarray(np.asarray(bits, dtype=np.int32))
         nopen = int((b < 0).sum())
