@@ -502,11 +502,12 @@ def main():
     ap.add_argument("--oracle-sample", type=int, default=16,
                     help="uneven-batch circuits (the cheapest) checked against the FP64 oracle")
     ap.add_argument("--uneven-circuits", type=int, default=256, help="circuits of the uneven batch")
-    ap.add_argument("--serve-circuits", type=int, default=96, help="brickwork circuits in the serving stream")
-    ap.add_argument("--serve-grid", type=int, default=1, help="6x6 grid circuits in the serving stream")
+    ap.add_argument("--serve-circuits", type=int, default=256, help="brickwork circuits in the serving stream")
+    ap.add_argument("--serve-grid", type=int, default=2, help="6x6 grid circuits in the serving stream")
     ap.add_argument("--serve-grid-cycles", type=int, default=4)
-    ap.add_argument("--serve-rate", type=float, default=10.0, help="Poisson arrival rate, circuits/s")
-    ap.add_argument("--serve-big-lane", type=int, default=0,
+    ap.add_argument("--serve-rate", type=float, default=40.0,
+                    help="Poisson arrival rate, circuits/s (above both paths' capacity: sustained throughput)")
+    ap.add_argument("--serve-big-lane", type=int, default=512,
                     help="serving: circuits whose bond can reach this run on the background lane (0 = off)")
     ap.add_argument("--lq-min-rank", type=int, default=-1,
                     help="Jacobi on the double-LQ factor for min(m,n) >= this (0 off; default: engine default)")
@@ -669,6 +670,7 @@ def main():
         lat, span, _ = serve_batched(eng, jobs, arrivals, args.precision)
         eng.set_tuning(**base)
         grid_lat = [float(lat[k]) for k, j in enumerate(jobs) if j["kind"] == "grid"]
+        brick_lat = np.array([lat[k] for k, j in enumerate(jobs) if j["kind"] != "grid"])
         log(f"  batched {args.precision}: {len(jobs) / span:.2f} circuits/s, p50 {np.percentile(lat, 50):.0f} ms, "
             f"p99 {np.percentile(lat, 99):.0f} ms, grid {grid_lat}")
         out["latency_ms_p50"] = round(float(np.percentile(lat, 50)), 2)
@@ -681,7 +683,16 @@ def main():
                    "big_lane_min_chi": args.serve_big_lane,
                    "circuits_per_s": round(len(jobs) / span, 3),
                    "latency_ms_p50": out["latency_ms_p50"], "latency_ms_p99": out["latency_ms_p99"],
-                   "grid_latency_ms": [round(x, 1) for x in grid_lat]}
+                   "grid_latency_ms": [round(x, 1) for x in grid_lat],
+                   "brickwork_latency_ms_p50": round(float(np.percentile(brick_lat, 50)), 2),
+                   "brickwork_latency_ms_p99": round(float(np.percentile(brick_lat, 99)), 2)}
+        if args.serve_grid:  # one grid circuit alone on the same path, for its latency in the mix
+            g0 = next(j for j in jobs if j["kind"] == "grid")
+            t0 = time.perf_counter()
+            st_ = mps.MpsState(eng, g0["n"], cutoff=CUTOFF, max_bond=g0["chi"], precision=args.precision).run(g0["gates"])
+            st_.expval_z_all()
+            st_.close()
+            serving["grid_alone_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
         if not args.no_plain:
             # the plain path serves one circuit at a time, so its latencies follow from each
             # circuit's measured plain run time through the FIFO recurrence
