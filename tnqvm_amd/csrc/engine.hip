@@ -184,6 +184,10 @@ struct hipmps_engine_s {
   int inner_sweeps = 1;
   int debug_sweeps = 0;  // HIPMPS_DEBUG_SWEEPS: per-sweep rotated / visited block pairs on stderr
   int lq_min_rank = 64;  // Jacobi on the double-LQ factor when min(m, n) >= this (0: never)
+  // LQ with look-ahead (next panel factored in LDS on one stream while the trailing rows update
+  // on another) for rows of at most LQ_REG_MAX_COLS columns; HIPMPS_LQ_LOOKAHEAD=0 turns it off
+  int lq_lookahead = 1;
+  hipEvent_t lq_ev[2 * MAX_LANES] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   DevBuf ws;     // per-wave arena
   DevBuf desc;   // task descriptors
   DevBuf small;  // counters / K / stats
@@ -690,11 +694,18 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     max_k = std::max(max_k, it.kub);
   }
   // double-LQ preconditioning: X = L Q~, L^H = L_Z Q_Z, Jacobi on R2 = L_Z^H
-  int nlq = 0, lq_max_r = 0, lq_max_rpad = 0, lq_max_c = 0;
+  // LQ tasks of items whose rows fit the look-ahead LQ (c <= LQ_REG_MAX_COLS, so r too) come first
+  // (nlq_a of them): the method is chosen per item, so that an item's result does not depend on
+  // the other items of its wave
+  int nlq = 0, nlq_a = 0, lq_max_r = 0, lq_max_rpad = 0, lq_max_c = 0;
   bool any_cut = false;
-  for (int q = 0; q < N; ++q) {
+  std::vector<int> lq_order;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int q = 0; q < N; ++q)
+      if (items[q].lq && ((items[q].c <= LQ_REG_MAX_COLS) == (pass == 0))) lq_order.push_back(q);
+  for (int q : lq_order) {
     auto& it = items[q];
-    if (!it.lq) continue;
+    if (it.c <= LQ_REG_MAX_COLS) ++nlq_a;
     const size_t panels = (it.r + 15) / 16;
     double2* Y = (double2*)(ws + it.off_y);
     double2* Z = (double2*)(ws + it.off_z);
@@ -802,15 +813,45 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     }
     return 0;
   };
+  // one LQ of a lane's slice: with look-ahead on the lane's stream and a partner stream when the
+  // rows fit (lanes 0, 1 pair with streams 2, 3), else panel by panel on the lane's stream
+  const bool lookahead = E->lq_lookahead && 2 * lq_lanes <= hipmps_engine_s::MAX_LANES && E->lq_ev[0] != nullptr;
+  // the LQ of tasks [b, b + n) of one array (second = 0: X, 1: L^H): look-ahead for the tasks of
+  // items in the first group (columns <= LQ_REG_MAX_COLS), panel by panel for the others
+  auto run_lq = [&](const LqTask* tasks, int b, int n, int second, int l) -> int {
+    const hipStream_t ls = E->lane_stream[l];
+    const int a_end = std::min(b + n, nlq_a), b_beg = std::max(b, nlq_a);
+    const bool la = lookahead && E->lane_stream[l + lq_lanes] != nullptr;
+    auto extent = [&](int from, int to, int& rows, int& cols) {
+      rows = cols = 0;
+      for (int k = from; k < to; ++k) {
+        const auto& it = items[lq_order[k]];
+        rows = std::max(rows, it.r);
+        cols = std::max(cols, second ? it.r : it.c);
+      }
+    };
+    int rows, cols;
+    if (a_end > b) {
+      extent(b, a_end, rows, cols);
+      if (la) HIP_TRY(launch_lq_lookahead(tasks + b, a_end - b, rows, cols, ls, E->lane_stream[l + lq_lanes],
+                                         E->lq_ev[2 * l], E->lq_ev[2 * l + 1]));
+      else HIP_TRY(launch_lq(tasks + b, a_end - b, rows, ls));
+    }
+    if (b + n > b_beg) {
+      extent(b_beg, b + n, rows, cols);
+      HIP_TRY(launch_lq(tasks + b_beg, b + n - b_beg, rows, ls));
+    }
+    return 0;
+  };
   if (nlq > 0) {
     RC_TRY(fork_lanes(lq_lanes));
     for (int l = 0; l < lq_lanes; ++l) {
       int b, n;
       lq_slice(l, b, n);
       const hipStream_t ls = E->lane_stream[l];
-      HIP_TRY(launch_lq(dlq + b, n, lq_max_r, ls));
+      RC_TRY(run_lq(dlq, b, n, 0, l));
       HIP_TRY(launch_lower_to_upper_h(dtr + b, n, lq_max_r, lq_max_r, ls));
-      HIP_TRY(launch_lq(dlq + N + b, n, lq_max_r, ls));
+      RC_TRY(run_lq(dlq + N, b, n, 1, l));
       HIP_TRY(launch_lower_to_upper_h(dtr + N + b, n, lq_max_r, lq_max_rpad, ls));
     }
     RC_TRY(join_lanes(lq_lanes));
@@ -2227,6 +2268,13 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
   for (int l = 1; l < hipmps_engine_s::MAX_LANES; ++l)
     ok = ok && hipStreamCreateWithFlags(&e->lane_stream[l], hipStreamNonBlocking) == hipSuccess;
   for (auto& ev : e->lane_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+  bool lq_ok = true;
+  for (auto& ev : e->lq_ev) lq_ok = lq_ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+  if (!lq_ok) {
+    (void)hipGetLastError();
+    e->lq_lookahead = 0;
+  }
+  if (const char* v = getenv("HIPMPS_LQ_LOOKAHEAD")) e->lq_lookahead = atoi(v) != 0;
   if (!ok) {
     (void)hipGetLastError();
     e->jacobi_lanes = 1;
@@ -2269,6 +2317,8 @@ int hipmps_engine_destroy(hipmps_engine e) {
   for (auto& ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : e->lane_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto& ev : e->lq_ev)
     if (ev) (void)hipEventDestroy(ev);
   for (int l = 1; l < hipmps_engine_s::MAX_LANES; ++l)
     if (e->lane_stream[l]) {

@@ -153,6 +153,13 @@ hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pai
                                int inner_sweeps, hipStream_t s);
 // Householder LQ of every task's M (panel by panel, one launch per panel).
 hipError_t launch_lq(const LqTask* d_tasks, int ntasks, int max_rows, hipStream_t s);
+// The same LQ with look-ahead on two streams: on sa, the next panel's rows take the current block
+// reflector and the panel is factored with its rows in registers; on sb, the trailing rows take the
+// block reflector meanwhile (events ea, eb order the two).  Rows of at most LQ_REG_MAX_COLS columns
+// (hipErrorNotSupported otherwise); on return every later launch on sa follows all of it.
+constexpr int LQ_REG_MAX_COLS = 512;
+hipError_t launch_lq_lookahead(const LqTask* d_tasks, int ntasks, int max_rows, int max_cols, hipStream_t sa,
+                               hipStream_t sb, hipEvent_t ea, hipEvent_t eb);
 // Tgt <- Tgt Q~ (Q~ = H_{rows-1} ... H_0 restricted to the first rows of M's row space, applied
 // panel by panel from the last), i.e. [Y | 0] Q~ for Tgt = [Y | 0].
 hipError_t launch_lq_apply_q(const LqTask* d_tasks, int ntasks, int max_rows, int max_rows_t, hipStream_t s);

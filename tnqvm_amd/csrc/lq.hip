@@ -312,6 +312,166 @@ __global__ __launch_bounds__(256) void k_widen(const TriTask* tasks) {
   T.dst[e] = j < T.n ? T.src[(size_t)i * T.ld_src + j] : make_double2(0, 0);
 }
 
+// ---- factorisation with the panel in registers, look-ahead on two streams ----------------------
+// Factor rows [k0, k1) of T.M (panel p) over columns [k0, cols) with the panel in LDS (16 rows of
+// at most LQ_REG_MAX_COLS columns, 128 KiB): the same reflectors as lq_factor_panel, with every
+// dependent access of the 16 serial steps in LDS instead of memory.  Eight waves; wave w updates
+// the rows i = w (mod 8), its inner products reduced within the wave.
+__global__ __launch_bounds__(512) void k_lq_factor_lds(const LqTask* tasks, int p) {
+  constexpr int NT = 512, NW = NT / 64;
+  __shared__ double2 sy[LQB][LQ_REG_MAX_COLS];
+  __shared__ double2 sG[LQB][LQB + 1];
+  __shared__ double2 sT[LQB][LQB + 1];
+  __shared__ double2 s_w0[LQB];
+  __shared__ double2 s_par[2];  // beta, w0
+  __shared__ double s_scale;
+  const LqTask& T = tasks[blockIdx.x];
+  const int k0 = p * LQB;
+  if (k0 >= T.rows) return;
+  const int k1 = min(k0 + LQB, T.rows), nr = k1 - k0;
+  const int W = T.cols - k0;  // columns k0 .. cols - 1 (local index c = column - k0)
+  double2* M = T.M;
+  const size_t ld = T.ld;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int e = tid; e < LQB * W; e += NT) {
+    const int i = e / W, c = e % W;
+    sy[i][c] = i < nr ? M[(size_t)(k0 + i) * ld + k0 + c] : make_double2(0, 0);
+  }
+  __syncthreads();
+  for (int r = 0; r < nr; ++r) {
+    if (wave == 0) {
+      double s = 0.0;
+      for (int c = r + 1 + lane; c < W; c += 64) s += cabs2(sy[r][c]);
+      s = wave_sum(s);
+      if (lane == 0) {
+        const double2 a = sy[r][r];
+        const double nrm = sqrt(cabs2(a) + s);
+        double2 beta = make_double2(0, 0), w0 = make_double2(0, 0);
+        double scale = 0.0;
+        if (nrm > 0.0) {
+          const double aa = sqrt(cabs2(a));
+          const double2 ph = aa > 0.0 ? cscale(a, 1.0 / aa) : make_double2(1.0, 0.0);
+          beta = cscale(ph, -nrm);  // -e^{i arg a} |x|: no cancellation in a - beta
+          const double2 v0 = make_double2(a.x - beta.x, a.y - beta.y);
+          scale = 1.0 / sqrt(cabs2(v0) + s);
+          w0 = cscale(v0, scale);
+        }
+        s_par[0] = beta;
+        s_par[1] = w0;
+        s_scale = scale;
+      }
+    }
+    __syncthreads();
+    const double2 w0 = s_par[1];
+    const double scale = s_scale;
+    // rows below: y -= 2 <y, w> w, w = (w0 at c = r, scale * row r beyond)
+    for (int i = r + 1 + wave; i < nr; i += NW) {
+      double2 d = lane == 0 ? cfmac(w0, sy[i][r], make_double2(0, 0)) : make_double2(0, 0);
+      for (int c = r + 1 + lane; c < W; c += 64) d = cfmac(cscale(sy[r][c], scale), sy[i][c], d);
+      d = wave_sum2(d);
+      const double2 m2d = cscale(d, -2.0);
+      if (lane == 0) sy[i][r] = cfma(m2d, w0, sy[i][r]);
+      for (int c = r + 1 + lane; c < W; c += 64) sy[i][c] = cfma(m2d, cscale(sy[r][c], scale), sy[i][c]);
+    }
+    __syncthreads();  // every read of row r done
+    for (int c = r + 1 + tid; c < W; c += NT) sy[r][c] = cscale(sy[r][c], scale);
+    if (tid == 0) {
+      sy[r][r] = s_par[0];
+      s_w0[r] = w0;
+    }
+    __syncthreads();
+  }
+  // G_ij = <w_i, w_j> = sum_l conj(w_j[l]) w_i[l] for i < j, support from local column j on
+  for (int pr = wave; pr < LQB * LQB; pr += NW) {
+    const int i = pr / LQB, j = pr % LQB;
+    if (j <= i) continue;
+    double2 d = make_double2(0, 0);
+    if (j < nr) {
+      if (lane == 0) d = cfmac(s_w0[j], sy[i][j], d);
+      for (int c = j + 1 + lane; c < W; c += 64) d = cfmac(sy[j][c], sy[i][c], d);
+    }
+    d = wave_sum2(d);
+    if (lane == 0) sG[i][j] = d;
+  }
+  __syncthreads();
+  for (int j = 0; j < LQB; ++j) {
+    if (tid < LQB) {
+      double2 t = make_double2(0, 0);
+      if (tid < j) {
+        for (int m = tid; m < j; ++m) t = cfma(sT[tid][m], sG[m][j], t);
+        t = cscale(t, -2.0);
+      } else if (tid == j) {
+        t = make_double2(k0 + j < k1 ? 2.0 : 0.0, 0.0);
+      }
+      sT[tid][j] = t;
+    }
+    __syncthreads();
+  }
+  double2* Tp = T.T + (size_t)p * LQB * LQB;
+  for (int e = tid; e < LQB * LQB; e += NT) Tp[e] = sT[e / LQB][e % LQB];
+  if (tid < nr) T.W0[k0 + tid] = s_w0[tid];
+  for (int e = tid; e < nr * W; e += NT) {
+    const int i = e / W, c = e % W;
+    M[(size_t)(k0 + i) * ld + k0 + c] = sy[i][c];
+  }
+}
+
+// Rows of panel p + 1 (16 rows from k1) with the block reflector of panel p
+__global__ __launch_bounds__(256) void k_lq_next(const LqTask* tasks, int p) {
+  __shared__ double2 sP[LQB][LQB + 1];
+  __shared__ double2 sQ[LQB][LQB + 1];
+  __shared__ double2 sT[LQR][LQB + 1];
+  const LqTask& T = tasks[blockIdx.x];
+  const int k0 = p * LQB;
+  const int k1 = min(k0 + LQB, T.rows);
+  if (k1 >= T.rows) return;
+  lq_apply_panel(T, k0, T.M, T.ld, k1, min(k1 + LQB, T.rows), 0, sP, sQ, sT);
+}
+
+// Rows beyond panel p + 1 with the block reflector of panel p, 16 rows per workgroup
+__global__ __launch_bounds__(256) void k_lq_rest(const LqTask* tasks, int p) {
+  __shared__ double2 sP[LQB][LQB + 1];
+  __shared__ double2 sQ[LQB][LQB + 1];
+  __shared__ double2 sT[LQR][LQB + 1];
+  const LqTask& T = tasks[blockIdx.y];
+  const int k0 = p * LQB;
+  const int r0 = k0 + 2 * LQB + LQR * (int)blockIdx.x;
+  if (r0 >= T.rows) return;
+  lq_apply_panel(T, k0, T.M, T.ld, r0, min(r0 + LQR, T.rows), 0, sP, sQ, sT);
+}
+
+hipError_t launch_lq_lookahead(const LqTask* d_tasks, int ntasks, int max_rows, int max_cols, hipStream_t sa,
+                               hipStream_t sb, hipEvent_t ea, hipEvent_t eb) {
+  if (ntasks <= 0 || max_rows <= 0) return hipSuccess;
+  if (max_cols > LQ_REG_MAX_COLS) return hipErrorNotSupported;
+  auto factor = [&](int p) { hipLaunchKernelGGL(k_lq_factor_lds, dim3(ntasks), dim3(512), 0, sa, d_tasks, p); };
+  hipError_t e = hipSuccess;
+  // sb starts after what sa has queued so far (the matrices are ready there)
+  if ((e = hipEventRecord(ea, sa)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(sb, ea, 0)) != hipSuccess) return e;
+  factor(0);
+  if ((e = hipEventRecord(ea, sa)) != hipSuccess) return e;
+  const int panels = (max_rows + LQB - 1) / LQB;
+  for (int p = 0; p + 1 < panels; ++p) {
+    // sa: rows of panel p + 1 need the rest-update of panel p - 1 (recorded in eb), sb: the rest
+    // update of panel p needs panel p's reflectors (recorded in ea); both waits are enqueued before
+    // the events are recorded again below
+    if (p > 0 && (e = hipStreamWaitEvent(sa, eb, 0)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(sb, ea, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lq_next, dim3(ntasks), dim3(256), 0, sa, d_tasks, p);
+    factor(p + 1);
+    if ((e = hipEventRecord(ea, sa)) != hipSuccess) return e;
+    const int trailing = max_rows - (p + 2) * LQB;
+    if (trailing > 0)
+      hipLaunchKernelGGL(k_lq_rest, dim3((trailing + LQR - 1) / LQR, ntasks), dim3(256), 0, sb, d_tasks, p);
+    if ((e = hipEventRecord(eb, sb)) != hipSuccess) return e;
+  }
+  // everything after this on sa sees both streams' work
+  if ((e = hipEventRecord(eb, sb)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(sa, eb, 0)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
 hipError_t launch_widen(const TriTask* d_tasks, int ntasks, int max_cols, int max_rows_dst, hipStream_t s) {
   if (ntasks <= 0) return hipSuccess;
   const size_t total = (size_t)max_cols * max_rows_dst;
