@@ -5,7 +5,7 @@
 Products (git-ignored, shipped with the tree):
   tnqvm_amd/libhipmps.so       HIP kernels + engine + C ABI (include/hipmps.h)
   tnqvm_amd/libtnqvm_hipmps.so C++ TNQVM visitor layer (hip-mps) on top of libhipmps
-  tests/cpp/bin/*              C++ visitor tests
+  tests/cpp/bin/*              C++ visitor tests, and their ASan/UBSan builds (*_san)
 """
 from __future__ import annotations
 
@@ -90,10 +90,33 @@ def build_cpp_tests() -> list:
     return out
 
 
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined", "-g", "-O1"]
+
+
+def build_sanitized() -> list:
+    """Host code under AddressSanitizer + UndefinedBehaviorSanitizer (g++, no device code): the
+    visitor layer with the C++ visitor tests (their CPU cases), and the CPU oracle with a driver
+    of its own (tests/cpp/sanitize/oracle_driver.cpp).  libhipmps itself is linked uninstrumented."""
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    bdir = os.path.join(tdir, "bin")
+    os.makedirs(bdir, exist_ok=True)
+    odir = os.path.join(ROOT, "oracle")
+    host = [os.path.join(HOST, f) for f in sorted(os.listdir(HOST)) if f.endswith(".cpp")]
+    vis = os.path.join(bdir, "visitor_tests_san")
+    _run(["g++", "-std=c++17"] + SAN_FLAGS + ["-I" + os.path.join(ROOT, "include"), "-I" + HOST, "-o", vis,
+          os.path.join(tdir, "visitor_tests.cpp")] + host +
+         ["-L" + PKG, "-lhipmps", "-L" + odir, "-loracle", "-Wl,-rpath," + PKG, "-Wl,-rpath," + odir])
+    orc = os.path.join(bdir, "oracle_san")
+    _run(["g++", "-std=c++17", "-fopenmp"] + SAN_FLAGS + ["-o", orc,
+          os.path.join(tdir, "sanitize", "oracle_driver.cpp"), os.path.join(odir, "mps_oracle.cpp")])
+    return [vis, orc]
+
+
 def build_all(jobs: int = 4) -> None:
     build_hipmps(jobs)
     if build_visitor():
         build_cpp_tests()
+        build_sanitized()
 
 
 if __name__ == "__main__":
