@@ -16,23 +16,33 @@ over the whole timed span.  The remaining layers (at least one) run untimed.
 Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
   plain_fp64_value   the same layers, one circuit at a time, complex128 everywhere
   speedup_vs_plain   value / plain_fp64_value
+  plain_fp64_gate_at_a_time_value
+                     plain FP64, one gate per call with a synchronisation after each
   uneven_batch       256 brickwork circuits of uneven size (qubits U{16..48}, depth
                      U{10..60}, chi_max from {32,64,128,256}) submitted together, against
                      each circuit run alone in plain FP64; circuits/s, applies/s, the ratio,
                      and the <Z> difference between the two paths per circuit
-  serving            a Poisson arrival stream of such circuits and a 6x6 grid fSim circuit
-                     (chi_max 1024) through the continuous-batching loop and through the
-                     plain path one at a time: circuits/s and p50/p99 latency for both
+  serving            Poisson arrivals (40/s) of those 256 brickwork circuits and two 6x6 grid
+                     fSim circuits (chi_max 1024) through the continuous-batching loop, circuits
+                     whose bond can reach 512 on the background lane; against the plain path
+                     one circuit at a time: circuits/s and p50/p99 latency for both, latency
+                     per class, and the grid circuit's time alone
   latency_ms_p50/p99 the batched serving latencies (arrival to result)
+  qaoa_60            the 60-qubit QAOA MaxCut circuit (3-regular graph, p = 4, odd-even swap
+                     network, chi_max 512, cutoff 1e-12), mixed and FP64: applies/s, energy,
+                     their difference, discarded weight
   roofline           the SVD (double LQ + Jacobi rounds) of the timed applies, FP64 bound:
-                     nominal 704 chi^3 flops per apply over its HIP-event time
+                     nominal 704 chi^3 flops per apply over its HIP-event time; the Jacobi
+                     rounds' executed flops and HBM traffic beside it
   noisy              the purified-MPS (noisy) path: a 40-qubit u3/CNOT circuit of depth 4 with the
                      ibmqx2 backend's single-qubit channels after every gate (qubit q takes the
                      channels of backend qubit q mod 5), no bond cap, cut 1e-9: wall time per
                      circuit with its <Z> read back; batch_64, 64 copies advanced together
-  cpu_baseline       the FP64 CPU oracle on the host cores at chi = 256; under "noisy", the
-                     restated purified-MPS algorithm on the same noisy circuit, with the largest
-                     <Z> difference from the GPU result
+  cpu_baseline       the exatn-mps step with a LAPACK SVD (numpy zgesdd) on the host cores at
+                     chi = 256, all cores and one thread, CPU model recorded; beside it the
+                     Jacobi oracle, the per-circuit oracle check of 16 uneven circuits (dense or
+                     oracle MPS, error per result) and, under "noisy", the restated
+                     purified-MPS algorithm on the same noisy circuit
 """
 from __future__ import annotations
 
@@ -576,7 +586,8 @@ def main():
         traffic_ratio = round(per_pair * prof["jacobi_pairs_visited"] / max(1.0, applies * 2 * 512 * 512 * 16), 2)
     roofline = {"bound": "mfma", "achieved": round(svd_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(svd_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                "kernel": "SVD of theta' 512x512: k_lq_panel0/k_lq_update (double LQ), k_jacobi_round, k_lq_apply_q",
+                "kernel": "SVD of theta' 512x512: double LQ (k_lq_factor_lds/k_lq_next/k_lq_rest with look-ahead, "
+                          "k_lq_panel0/k_lq_update), k_jacobi_round, k_lq_apply_q",
                 "flops_per_unit": NOMINAL_SVD * CHI ** 3, "unit_of_work": "one chi=256 apply's SVD (nominal 704 chi^3)",
                 "avg_wave_ms": round(svd_ms / waves, 3), "svds_per_wave": round(applies / waves, 1),
                 "timing": "HIP events on the engine stream around the LQ and Jacobi phases of each wave",

@@ -178,3 +178,50 @@ def test_sweep_cap_surfaces_convergence(eng):
     s.flush()
     assert s.stats()["svds_nonconverged"] == 0
     s.close()
+
+
+def test_serve_admission(eng):
+    """Serving admission by projected footprint: hipmps_serve_submit refuses a circuit whose
+    footprint exceeds the engine's HBM budget (HIPMPS_E_NOMEM, nothing queued); two circuits that
+    fit only one at a time are both accepted, the second waits in the queue until the first has
+    finished, and both give their alone results."""
+    a = {"n": 14, "gates": C.haar_brickwork(14, 10, seed=0x7E5A0108), "chi": 64}
+    b = {"n": 14, "gates": C.haar_brickwork(14, 12, seed=0x7E5A0109), "chi": 64}
+    big = {"n": 16, "gates": C.haar_brickwork(16, 12, seed=0x7E5A010A), "chi": 128}
+    jobs = _jobs([a, b, big])
+    want = [_alone(eng, j) for j in jobs[:2]]
+    fps = [eng.footprint(j["n"], j["gates"], j["trunc"]) for j in jobs]
+    budget = max(fps[:2]) + (1 << 16)
+    assert sum(fps[:2]) > budget and fps[2] > budget
+    say("serve budget", budget, "footprints", fps)
+    e2 = mps.Engine(hbm_budget_bytes=budget)
+    try:
+        srv = mps.Server(e2)
+        with pytest.raises(_capi.HipMpsError) as ei:
+            srv.submit(jobs[2]["n"], jobs[2]["gates"], jobs[2]["trunc"], zz=jobs[2]["zz"])
+        assert ei.value.code == _capi.HIPMPS_E_NOMEM
+        assert srv.pending() == (0, 0)
+        tickets = [srv.submit(j["n"], j["gates"], j["trunc"], zz=j["zz"]) for j in jobs[:2]]
+        srv.step()
+        assert srv.pending() == (1, 1)  # the second waits for the first's memory
+        results = {}
+        for _ in range(100000):
+            if len(results) == 2:
+                break
+            srv.step()
+            for t in tickets:
+                if t not in results:
+                    r = srv.poll(t)
+                    if r is not None:
+                        results[t] = r
+                        if len(results) == 1:
+                            assert t == tickets[0]
+        assert len(results) == 2
+        for t, w in zip(tickets, want):
+            r = results[t]
+            assert r["status"] == 0
+            assert np.abs(r["expval_z"] - w["expval_z"]).max() < 1e-12
+            assert np.abs(r["expval_zz"] - w["expval_zz"]).max() < 1e-12
+            assert r["max_bond"] == w["max_bond"]
+    finally:
+        e2.close()

@@ -110,6 +110,29 @@ def test_sv_sampling_determinism():
     assert set(sv.sample(1, 10, [2, 0])) == {"10"}
 
 
+def test_sv_sampling_inverse_cdf():
+    """GenerateSamples (GateMatrixAlgebra.hpp:125-156) is inverse-CDF sampling with sorted draws:
+    shot j is the first basis index x whose cumulative probability exceeds the j-th smallest of
+    the shots' mt19937_64 uniforms.  Pinned on a product state Ry(t_q)|0>, whose probabilities
+    are products of cos^2 and sin^2 of t_q / 2, with the draws from an independent MT19937-64."""
+    n, shots, seed = 4, 300, 2024
+    th = [0.4, 1.3, 2.2, 2.8]
+    sv = O.StateVector(n).run([{"name": "Ry", "qubits": (q,), "params": (t,)} for q, t in enumerate(th)])
+    meas = [2, 0, 3]
+    got = sv.sample(seed, shots, meas)
+    p = np.array([np.prod([np.sin(th[q] / 2) ** 2 if (x >> q) & 1 else np.cos(th[q] / 2) ** 2 for q in range(n)])
+                  for x in range(1 << n)])
+    cdf = np.cumsum(p)
+    gen = _Mt64(seed)
+    draws = sorted(gen.uniform() for _ in range(shots))
+    want = []
+    for r in draws:
+        x = int(np.searchsorted(cdf, r, side="right"))  # first x with cdf[x] > r
+        if x < (1 << n):
+            want.append("".join("1" if (x >> q) & 1 else "0" for q in meas))
+    assert got == want
+
+
 # ---------------------------------------------------------------- SVD vs LAPACK
 def test_svd_golden_spectra():
     z = np.load(os.path.join(GOLD, "svd_spectra.npz"))

@@ -330,9 +330,15 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
       v1 = make_double2(0, 0);
     }
   };
+  // Complex products on real MFMA with three real products each instead of four:
+  //  * diagonal tiles (Hermitian): Re = Ar Ar^T + Ai Ai^T, and Im = X - X^T with X = Ar Ai^T;
+  //  * off-diagonal tile conj(A) B^T (Gauss): K1 = (Ar - Ai) Br^T, K2 = Ar (Bi - Br)^T,
+  //    K3 = -Ai (Br + Bi)^T;  Re = K1 - K3, Im = K1 + K2.
+  // Each partial sum of the four waves goes to LDS as (Re, X) for the diagonal tiles and
+  // (Re, Im) for the off-diagonal one; Im of the diagonal tiles is formed after the sum.
   if (!T.gram32) {
-    mfma_d4 a00r = {0, 0, 0, 0}, a00i = {0, 0, 0, 0}, a01r = {0, 0, 0, 0}, a01i = {0, 0, 0, 0};
-    mfma_d4 a11r = {0, 0, 0, 0}, a11i = {0, 0, 0, 0};
+    mfma_d4 a00r = {0, 0, 0, 0}, a00x = {0, 0, 0, 0}, a11r = {0, 0, 0, 0}, a11x = {0, 0, 0, 0};
+    mfma_d4 k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
     double2 x0[PD], x1[PD];
 #pragma unroll
     for (int d = 0; d < PD; ++d) ld_unit(wave + 4 * d, x0[d], x1[d]);
@@ -342,17 +348,14 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
         const double2 v0 = x0[d], v1 = x1[d];
         ld_unit(u0 + 4 * (d + PD), x0[d], x1[d]);
         a00r = mfma64(v0.x, v0.x, a00r);
-        a01r = mfma64(v0.x, v1.x, a01r);
         a11r = mfma64(v1.x, v1.x, a11r);
-        a00i = mfma64(v0.x, v0.y, a00i);
-        a01i = mfma64(v0.x, v1.y, a01i);
-        a11i = mfma64(v1.x, v1.y, a11i);
+        a00x = mfma64(v0.x, v0.y, a00x);
+        a11x = mfma64(v1.x, v1.y, a11x);
+        k1 = mfma64(v0.x - v0.y, v1.x, k1);
+        k2 = mfma64(v0.x, v1.y - v1.x, k2);
+        k3 = mfma64(-v0.y, v1.x + v1.y, k3);
         a00r = mfma64(v0.y, v0.y, a00r);
-        a01r = mfma64(v0.y, v1.y, a01r);
         a11r = mfma64(v1.y, v1.y, a11r);
-        a00i = mfma64(-v0.y, v0.x, a00i);
-        a01i = mfma64(-v0.y, v1.x, a01i);
-        a11i = mfma64(-v1.y, v1.x, a11i);
       }
     }
     // f64 D[i][j]: lane j + 16 (i % 4), register i / 4
@@ -361,9 +364,9 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = 4 * v + k4, j = l16;
-          double2 g00 = make_double2(a00r[v], a00i[v]);
-          double2 g01 = make_double2(a01r[v], a01i[v]);
-          double2 g11 = make_double2(a11r[v], a11i[v]);
+          double2 g00 = make_double2(a00r[v], a00x[v]);
+          double2 g01 = make_double2(k1[v] - k3[v], k1[v] + k2[v]);
+          double2 g11 = make_double2(a11r[v], a11x[v]);
           if (w) {
             g00 = cadd(g00, sG[i][j]);
             g01 = cadd(g01, sG[i][JB + j]);
@@ -378,8 +381,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     }
   } else {
     typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-    mfma_f4 b00r = {0, 0, 0, 0}, b00i = {0, 0, 0, 0}, b01r = {0, 0, 0, 0}, b01i = {0, 0, 0, 0};
-    mfma_f4 b11r = {0, 0, 0, 0}, b11i = {0, 0, 0, 0};
+    mfma_f4 b00r = {0, 0, 0, 0}, b00x = {0, 0, 0, 0}, b11r = {0, 0, 0, 0}, b11x = {0, 0, 0, 0};
+    mfma_f4 h1 = {0, 0, 0, 0}, h2 = {0, 0, 0, 0}, h3 = {0, 0, 0, 0};
     double2 x0[PD], x1[PD];
 #pragma unroll
     for (int d = 0; d < PD; ++d) ld_unit(wave + 4 * d, x0[d], x1[d]);
@@ -389,17 +392,14 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
         const float p0 = (float)x0[d].x, q0 = (float)x0[d].y, p1 = (float)x1[d].x, q1 = (float)x1[d].y;
         ld_unit(u0 + 4 * (d + PD), x0[d], x1[d]);
         b00r = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, p0, b00r, 0, 0, 0);
-        b01r = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, p1, b01r, 0, 0, 0);
         b11r = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, p1, b11r, 0, 0, 0);
-        b00i = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, q0, b00i, 0, 0, 0);
-        b01i = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, q1, b01i, 0, 0, 0);
-        b11i = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, q1, b11i, 0, 0, 0);
+        b00x = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, q0, b00x, 0, 0, 0);
+        b11x = __builtin_amdgcn_mfma_f32_16x16x4f32(p1, q1, b11x, 0, 0, 0);
+        h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(p0 - q0, p1, h1, 0, 0, 0);
+        h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(p0, q1 - p1, h2, 0, 0, 0);
+        h3 = __builtin_amdgcn_mfma_f32_16x16x4f32(-q0, p1 + q1, h3, 0, 0, 0);
         b00r = __builtin_amdgcn_mfma_f32_16x16x4f32(q0, q0, b00r, 0, 0, 0);
-        b01r = __builtin_amdgcn_mfma_f32_16x16x4f32(q0, q1, b01r, 0, 0, 0);
         b11r = __builtin_amdgcn_mfma_f32_16x16x4f32(q1, q1, b11r, 0, 0, 0);
-        b00i = __builtin_amdgcn_mfma_f32_16x16x4f32(-q0, p0, b00i, 0, 0, 0);
-        b01i = __builtin_amdgcn_mfma_f32_16x16x4f32(-q0, p1, b01i, 0, 0, 0);
-        b11i = __builtin_amdgcn_mfma_f32_16x16x4f32(-q1, p1, b11i, 0, 0, 0);
       }
     }
     // f32 D[i][j]: lane j + 16 (i / 4), register i % 4; the four waves' partial tiles are
@@ -409,9 +409,9 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = 4 * k4 + v, j = l16;
-          double2 g00 = make_double2((double)b00r[v], (double)b00i[v]);
-          double2 g01 = make_double2((double)b01r[v], (double)b01i[v]);
-          double2 g11 = make_double2((double)b11r[v], (double)b11i[v]);
+          double2 g00 = make_double2((double)b00r[v], (double)b00x[v]);
+          double2 g01 = make_double2((double)h1[v] - (double)h3[v], (double)h1[v] + (double)h2[v]);
+          double2 g11 = make_double2((double)b11r[v], (double)b11x[v]);
           if (w) {
             g00 = cadd(g00, sG[i][j]);
             g01 = cadd(g01, sG[i][JB + j]);
@@ -426,8 +426,14 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     }
   }
   {
+    // Im of the diagonal tiles from X (X[i][j] - X[j][i]), and the lower off-diagonal tile
     const int i = tid >> 4, j = tid & 15;
-    sG[JB + i][j] = cconj(sG[j][JB + i]);
+    const double x00 = sG[i][j].y - sG[j][i].y, x11 = sG[JB + i][JB + j].y - sG[JB + j][JB + i].y;
+    const double2 g10 = cconj(sG[j][JB + i]);
+    __syncthreads();
+    sG[i][j].y = x00;
+    sG[JB + i][JB + j].y = x11;
+    sG[JB + i][j] = g10;
   }
   __syncthreads();
   // ---- convergence test of this block pair: max |G_ij|^2 / (G_ii G_jj) over i != j ----
@@ -552,7 +558,10 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
 
   // ---- apply Y' = W^T Y; wave owns column tiles wave, wave + 4, ... (all 32 rows) ----
   for (int ctile = wave; ctile < nt; ctile += 4) {
-    mfma_d4 r0 = {0, 0, 0, 0}, i0 = {0, 0, 0, 0}, r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
+    // (A + iB)(C + iD) with three real products (Gauss): K1 = (A + B) C, K2 = A (D - C),
+    // K3 = B (C + D); Re = K1 - K3, Im = K1 + K2
+    mfma_d4 e0 = {0, 0, 0, 0}, f0 = {0, 0, 0, 0}, g0 = {0, 0, 0, 0};
+    mfma_d4 e1 = {0, 0, 0, 0}, f1 = {0, 0, 0, 0}, g1 = {0, 0, 0, 0};
     const int ncol = (ctile + 4) * 16 + l16;
     const bool nok = ctile + 4 < nt && ncol < CT;
 #pragma unroll
@@ -562,21 +571,20 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
       const double2 w1 = sW[kk * 4 + k4][16 + l16];  // rows 16..31
       const int row = kk * 4 + k4;
       bq[kk] = nok ? Y[(size_t)(row < JB ? ba * JB + row : bb * JB + row - JB) * ld + ncol] : make_double2(0, 0);
-      r0 = mfma64(w0.x, b.x, r0);
-      r1 = mfma64(w1.x, b.x, r1);
-      i0 = mfma64(w0.x, b.y, i0);
-      i1 = mfma64(w1.x, b.y, i1);
-      r0 = mfma64(-w0.y, b.y, r0);
-      r1 = mfma64(-w1.y, b.y, r1);
-      i0 = mfma64(w0.y, b.x, i0);
-      i1 = mfma64(w1.y, b.x, i1);
+      const double dmc = b.y - b.x, cpd = b.x + b.y;
+      e0 = mfma64(w0.x + w0.y, b.x, e0);
+      e1 = mfma64(w1.x + w1.y, b.x, e1);
+      f0 = mfma64(w0.x, dmc, f0);
+      f1 = mfma64(w1.x, dmc, f1);
+      g0 = mfma64(w0.y, cpd, g0);
+      g1 = mfma64(w1.y, cpd, g1);
     }
     const int col = ctile * 16 + l16;
     if (col < CT) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        Y[(size_t)(ba * JB + 4 * v + k4) * ld + col] = make_double2(r0[v], i0[v]);
-        Y[(size_t)(bb * JB + 4 * v + k4) * ld + col] = make_double2(r1[v], i1[v]);
+        Y[(size_t)(ba * JB + 4 * v + k4) * ld + col] = make_double2(e0[v] - g0[v], e0[v] + f0[v]);
+        Y[(size_t)(bb * JB + 4 * v + k4) * ld + col] = make_double2(e1[v] - g1[v], e1[v] + f1[v]);
       }
     }
   }
