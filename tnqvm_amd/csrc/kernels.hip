@@ -482,10 +482,16 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   }
   __syncthreads();
   // ---- inner sweep (one barrier per step) ----
+  // G is held by slot: at every step, pair k is rows k and JP-1-k of the buffer.  Each step
+  // writes its rotated 2x2 blocks where the next step reads them (a row moves from slot x to
+  // x-1, slot 1 to JP-1, slot 0 stays), so every G address is fixed per thread; after JP-1
+  // steps the slots are back in row order.  W stays in row order (single buffer).
   const int bk = tid >> 4, bl = tid & 15;  // G block of rotation pairs bk, bl
   const int wi = tid >> 3, wl = tid & 7;   // W row wi, rotation pairs wl and wl + 8
   auto nxt = [](int x) { return x == JP - 1 ? 1 : x + 1; };  // row index one step later
-  int pk = bk, qk = JP - 1 - bk, pl = bl, ql = JP - 1 - bl;
+  auto prv = [](int x) { return x == 0 ? 0 : (x == 1 ? JP - 1 : x - 1); };  // slot one step later
+  const int pk = bk, qk = JP - 1 - bk, pl = bl, ql = JP - 1 - bl;
+  const int dpk = prv(pk), dqk = prv(qk), dpl = prv(pl), dql = prv(ql);
   int pw0 = wl, qw0 = JP - 1 - wl, pw1 = wl + 8, qw1 = JP - 1 - (wl + 8);
   // next-step pair owned by this thread (jn), and which row of pair bk (rra) and of pair
   // bl (rrb) its members are
@@ -532,10 +538,10 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     double2 sen;
     jacobi_rotation(alpha, beta, gam, itol, floor2, cn, sen);
     // ---- all LDS writes of the step ----
-    gB[pk][pl] = n00;
-    gB[pk][ql] = n01;
-    gB[qk][pl] = n10;
-    gB[qk][ql] = n11;
+    gB[dpk][dpl] = n00;
+    gB[dpk][dql] = n01;
+    gB[dqk][dpl] = n10;
+    gB[dqk][dql] = n11;
     sW[wi][pw0] = v0p;
     sW[wi][qw0] = v0q;
     sW[wi][pw1] = v1p;
@@ -546,10 +552,6 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     double2 (*tmp)[JP + 1] = gA;
     gA = gB;
     gB = tmp;
-    if (bk) pk = nxt(pk);
-    qk = nxt(qk);
-    if (bl) pl = nxt(pl);
-    ql = nxt(ql);
     if (wl) pw0 = nxt(pw0);
     qw0 = nxt(qw0);
     pw1 = nxt(pw1);
