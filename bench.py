@@ -22,12 +22,14 @@ Printed: ONE JSON line (rank 0, N = 1).  Fields beyond the driver contract:
                      U{10..60}, chi_max from {32,64,128,256}) submitted together, against
                      each circuit run alone in plain FP64; circuits/s, applies/s, the ratio,
                      and the <Z> difference between the two paths per circuit
-  serving            Poisson arrivals (40/s) of those 256 brickwork circuits and two 6x6 grid
-                     fSim circuits (chi_max 1024) through the continuous-batching loop, circuits
-                     whose bond can reach 512 on the background lane; against the plain path
-                     one circuit at a time: circuits/s and p50/p99 latency for both, latency
-                     per class, and the grid circuit's time alone
-  latency_ms_p50/p99 the batched serving latencies (arrival to result)
+  serving            Poisson arrivals (40/s, above what either path sustains) of those 256
+                     brickwork circuits and two 6x6 grid fSim circuits (chi_max 1024) through the
+                     continuous-batching loop, circuits whose bond can reach 512 on the background
+                     lane; against the plain path one circuit at a time: sustained circuits/s and
+                     p50/p99 latency for both, latency per class, the grid circuit's time alone
+  serving_moderate   the same loop below capacity: 96 of the brickwork circuits and one grid
+                     circuit at 5 arrivals/s
+  latency_ms_p50/p99 the batched latencies (arrival to result) of serving_moderate
   qaoa_60            the 60-qubit QAOA MaxCut circuit (3-regular graph, p = 4, odd-even swap
                      network, chi_max 512, cutoff 1e-12), mixed and FP64: applies/s, energy,
                      their difference, discarded weight
@@ -149,7 +151,7 @@ def serving_stream(brick, n_grid, grid_cycles, seed):
     jobs = [dict(j, src=k) for k, j in enumerate(brick)]
     for k in range(n_grid):
         g = C.route_nearest_neighbour(C.grid_fsim(6, 6, grid_cycles, seed=seed + 2 + k))
-        jobs.append({"n": 36, "gates": g, "chi": 1024, "kind": "grid", "src": -1})
+        jobs.append({"n": 36, "gates": g, "chi": 1024, "kind": "grid", "src": -1, "gid": k})
     return [jobs[i] for i in rng.permutation(len(jobs))]
 
 
@@ -495,6 +497,70 @@ def calibrate_copy(nbytes: int, reps: int = 4):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def serving_block(eng, args, brick, n_grid, rate, plain_own, plain_grid):
+    """Poisson arrivals at `rate` of the brickwork circuits `brick` (indices into the uneven batch
+    kept as "src") and `n_grid` 6x6 grid circuits through the continuous-batching loop, against
+    the plain FP64 path one circuit at a time (FIFO over each circuit's measured plain run time;
+    `plain_own` holds those of the uneven batch, `plain_grid` caches the grid circuits')."""
+    jobs = serving_stream(brick, n_grid, args.serve_grid_cycles, 0x7E5A0010)
+    arrivals = np.cumsum(np.random.default_rng(0x5E7).exponential(1.0 / rate, size=len(jobs)))
+    arrivals -= arrivals[0]
+    log(f"serving: {len(brick)} brickwork + {n_grid} 6x6 grid circuits, Poisson arrivals at {rate}/s")
+    base = eng.tuning()
+    eng.set_tuning(big_lane_min_chi=args.serve_big_lane)
+    lat, span, _ = serve_batched(eng, jobs, arrivals, args.precision)
+    eng.set_tuning(**base)
+    grid_lat = [float(lat[k]) for k, j in enumerate(jobs) if j["kind"] == "grid"]
+    brick_lat = np.array([lat[k] for k, j in enumerate(jobs) if j["kind"] != "grid"])
+    log(f"  batched {args.precision}: {len(jobs) / span:.2f} circuits/s, p50 {np.percentile(lat, 50):.0f} ms, "
+        f"p99 {np.percentile(lat, 99):.0f} ms, grid {grid_lat}")
+    out = {"workload": f"Poisson arrivals of {len(brick)} of the uneven brickwork circuits and {n_grid} 6x6 grid "
+                       f"fSim circuit(s) ({args.serve_grid_cycles} cycles, snake map, routed, chi_max 1024), "
+                       "svd-cutoff 1e-10",
+           "circuits": len(jobs), "brickwork": len(brick), "grid_6x6": n_grid,
+           "arrival_rate_per_s": rate, "precision": args.precision, "big_lane_min_chi": args.serve_big_lane,
+           "circuits_per_s": round(len(jobs) / span, 3),
+           "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+           "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+           "grid_latency_ms": [round(x, 1) for x in grid_lat],
+           "brickwork_latency_ms_p50": round(float(np.percentile(brick_lat, 50)), 2),
+           "brickwork_latency_ms_p99": round(float(np.percentile(brick_lat, 99)), 2)}
+    if n_grid and "grid_alone_ms" not in plain_grid:  # one grid circuit alone on the same path
+        g0 = next(j for j in jobs if j["kind"] == "grid")
+        t0 = time.perf_counter()
+        st_ = mps.MpsState(eng, g0["n"], cutoff=CUTOFF, max_bond=g0["chi"], precision=args.precision).run(g0["gates"])
+        st_.expval_z_all()
+        st_.close()
+        plain_grid["grid_alone_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+    if n_grid:
+        out["grid_alone_ms"] = plain_grid["grid_alone_ms"]
+    if not args.no_plain:
+        # the plain path serves one circuit at a time, so its latencies follow from each
+        # circuit's measured plain run time through the FIFO recurrence
+        service = np.zeros(len(jobs))
+        for k, j in enumerate(jobs):
+            key = ("grid", j["gid"]) if j["src"] < 0 else None
+            if j["src"] >= 0 and plain_own is not None:
+                service[k] = plain_own[j["src"]]
+            elif key in plain_grid:
+                service[k] = plain_grid[key]
+            else:
+                _, _, _, own = serve_plain(eng, [j], np.zeros(1))
+                service[k] = own[0]
+                if key is not None:
+                    plain_grid[key] = own[0]
+        plat, pspan = fifo_latency(arrivals, service)
+        log(f"  plain fp64: {len(jobs) / pspan:.2f} circuits/s, p50 {np.percentile(plat, 50):.0f} ms, "
+            f"p99 {np.percentile(plat, 99):.0f} ms")
+        out["plain_fp64"] = {"circuits_per_s": round(len(jobs) / pspan, 3),
+                             "latency_ms_p50": round(float(np.percentile(plat, 50)), 2),
+                             "latency_ms_p99": round(float(np.percentile(plat, 99)), 2),
+                             "method": "FIFO recurrence over the measured plain FP64 run time of each circuit (one "
+                                       "circuit at a time)"}
+        out["speedup_vs_plain"] = round(pspan / span, 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -517,6 +583,8 @@ def main():
     ap.add_argument("--serve-grid-cycles", type=int, default=4)
     ap.add_argument("--serve-rate", type=float, default=40.0,
                     help="Poisson arrival rate, circuits/s (above both paths' capacity: sustained throughput)")
+    ap.add_argument("--serve-rate-moderate", type=float, default=5.0,
+                    help="arrival rate of the second serving block (96 brickwork + 1 grid circuit)")
     ap.add_argument("--serve-big-lane", type=int, default=512,
                     help="serving: circuits whose bond can reach this run on the background lane (0 = off)")
     ap.add_argument("--lq-min-rank", type=int, default=-1,
@@ -671,59 +739,17 @@ def main():
                                   else None})
         out["uneven_batch"] = ub
     if not args.no_serving:
-        jobs = serving_stream(uneven[:args.serve_circuits], args.serve_grid, args.serve_grid_cycles, 0x7E5A0010)
-        arrivals = np.cumsum(np.random.default_rng(0x5E7).exponential(1.0 / args.serve_rate, size=len(jobs)))
-        arrivals -= arrivals[0]
-        log(f"serving: {args.serve_circuits} brickwork + {args.serve_grid} 6x6 grid circuits, Poisson arrivals "
-            f"at {args.serve_rate}/s")
-        base = eng.tuning()
-        eng.set_tuning(big_lane_min_chi=args.serve_big_lane)
-        lat, span, _ = serve_batched(eng, jobs, arrivals, args.precision)
-        eng.set_tuning(**base)
-        grid_lat = [float(lat[k]) for k, j in enumerate(jobs) if j["kind"] == "grid"]
-        brick_lat = np.array([lat[k] for k, j in enumerate(jobs) if j["kind"] != "grid"])
-        log(f"  batched {args.precision}: {len(jobs) / span:.2f} circuits/s, p50 {np.percentile(lat, 50):.0f} ms, "
-            f"p99 {np.percentile(lat, 99):.0f} ms, grid {grid_lat}")
-        out["latency_ms_p50"] = round(float(np.percentile(lat, 50)), 2)
-        out["latency_ms_p99"] = round(float(np.percentile(lat, 99)), 2)
-        serving = {"workload": f"Poisson arrivals of {args.serve_circuits} of the uneven brickwork circuits and "
-                               f"{args.serve_grid} 6x6 grid fSim circuit(s) ({args.serve_grid_cycles} cycles, snake "
-                               "map, routed, chi_max 1024), svd-cutoff 1e-10",
-                   "circuits": len(jobs), "brickwork": args.serve_circuits, "grid_6x6": args.serve_grid,
-                   "arrival_rate_per_s": args.serve_rate, "precision": args.precision,
-                   "big_lane_min_chi": args.serve_big_lane,
-                   "circuits_per_s": round(len(jobs) / span, 3),
-                   "latency_ms_p50": out["latency_ms_p50"], "latency_ms_p99": out["latency_ms_p99"],
-                   "grid_latency_ms": [round(x, 1) for x in grid_lat],
-                   "brickwork_latency_ms_p50": round(float(np.percentile(brick_lat, 50)), 2),
-                   "brickwork_latency_ms_p99": round(float(np.percentile(brick_lat, 99)), 2)}
-        if args.serve_grid:  # one grid circuit alone on the same path, for its latency in the mix
-            g0 = next(j for j in jobs if j["kind"] == "grid")
-            t0 = time.perf_counter()
-            st_ = mps.MpsState(eng, g0["n"], cutoff=CUTOFF, max_bond=g0["chi"], precision=args.precision).run(g0["gates"])
-            st_.expval_z_all()
-            st_.close()
-            serving["grid_alone_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
-        if not args.no_plain:
-            # the plain path serves one circuit at a time, so its latencies follow from each
-            # circuit's measured plain run time through the FIFO recurrence
-            need = [k for k, j in enumerate(jobs) if j["src"] < 0 or plain_own is None]
-            _, _, _, own_new = serve_plain(eng, [jobs[k] for k in need], np.zeros(len(need)))
-            service = np.zeros(len(jobs))
-            for k, j in enumerate(jobs):
-                if j["src"] >= 0 and plain_own is not None:
-                    service[k] = plain_own[j["src"]]
-            service[need] = own_new
-            plat, pspan = fifo_latency(arrivals, service)
-            log(f"  plain fp64: {len(jobs) / pspan:.2f} circuits/s, p50 {np.percentile(plat, 50):.0f} ms, "
-                f"p99 {np.percentile(plat, 99):.0f} ms")
-            serving["plain_fp64"] = {"circuits_per_s": round(len(jobs) / pspan, 3),
-                                     "latency_ms_p50": round(float(np.percentile(plat, 50)), 2),
-                                     "latency_ms_p99": round(float(np.percentile(plat, 99)), 2),
-                                     "method": "FIFO recurrence over the measured plain FP64 run time of each "
-                                               "circuit (one circuit at a time)"}
-            serving["speedup_vs_plain"] = round(pspan / span, 3)
+        plain_grid = {}  # plain FP64 run time of each grid circuit, by seed
+        serving = serving_block(eng, args, uneven[:args.serve_circuits], args.serve_grid, args.serve_rate,
+                                plain_own, plain_grid)
         out["serving"] = serving
+        # the same loop below capacity: 96 of the brickwork circuits and one grid circuit at a rate
+        # the batched path sustains, so that latency is service time rather than queueing
+        moderate = serving_block(eng, args, uneven[:96], min(1, args.serve_grid), args.serve_rate_moderate,
+                                 plain_own, plain_grid)
+        out["serving_moderate"] = moderate
+        out["latency_ms_p50"] = moderate["latency_ms_p50"]
+        out["latency_ms_p99"] = moderate["latency_ms_p99"]
     if not args.no_qaoa60:
         log("qaoa60: 60-qubit QAOA MaxCut p=4, swap network, chi_max 512, cutoff 1e-12")
         q = {}
