@@ -334,12 +334,12 @@ int projected_max_bond(const hipmps_circuit& c) {
 // HBM a circuit may hold while it runs, for admission: its sites at the largest bonds they can
 // reach, twice (a growing site holds its old buffer until theta has read it), plus its share of
 // the wave arena (theta', the Jacobi matrix and the LQ factors of up to n/2 gates of bond
-// chi: about 5 (2 chi)^2 complex128 each)
+// chi: about 6 (2 chi)^2 complex128 each, the tiled copy of the Jacobi matrix included)
 size_t circuit_footprint(const hipmps_circuit& c) {
   const int maxb = projected_max_bond(c);
   const size_t sites = projected_bytes(c.n_qubits, maxb, c.trunc.precision == HIPMPS_MIXED);
   const size_t gates = (size_t)std::max(1, c.n_qubits / 2);
-  return 2 * sites + gates * 5 * (size_t)(2 * maxb) * (size_t)(2 * maxb) * 16;
+  return 2 * sites + gates * 6 * (size_t)(2 * maxb) * (size_t)(2 * maxb) * 16;
 }
 
 // capacity a pure site grows to: doubled, but never past the largest the site can ever hold
@@ -512,6 +512,7 @@ struct WaveItem {
   size_t off_z, off_j, off_yt, off_lq;
   int ki, kj;                        // Kraus legs of the two sites (fused into cl and cr)
   size_t off_pin, off_pout;          // left site with its Kraus leg moved next to the left bond
+  size_t off_jt;                     // the Jacobi matrix in tiles during the sweeps
 };
 
 int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
@@ -552,6 +553,8 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     off = align_up(off + (size_t)it.r_pad * it.c * 16);
     it.off_vec = off;
     off = align_up(off + (size_t)it.r_pad * (8 * 5 + 4));
+    it.off_jt = off;  // r_pad x (columns of the Jacobi matrix rounded up to 16)
+    off = align_up(off + (size_t)it.r_pad * (((it.lq ? it.r : it.c) + 15) / 16 * 16) * 16);
     if (it.lq) {
       const size_t panels = (it.r + 15) / 16;  // LQ panels of 16 rows, T factor 16 x 16 each
       it.off_z = off;  // L^H, then the second LQ (r x r)
@@ -868,7 +871,17 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   }
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[1], st));
 
-  // ---- Jacobi sweeps ----
+  // ---- Jacobi sweeps, on a tiled copy of every Jacobi matrix ----
+  int jt_blocks = 0, jt_ntc = 0;
+  for (int q = 0; q < N; ++q) {
+    JacTask& J = hjt[q];
+    J.Yt = (double2*)(ws + items[q].off_jt);
+    J.ntc = (J.c_tot + 15) / 16;
+    jt_blocks = std::max(jt_blocks, J.r_pad / 16);
+    jt_ntc = std::max(jt_ntc, J.ntc);
+  }
+  HIP_TRY(hipMemcpyAsync(djt, hjt, N * sizeof(JacTask), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_to_tiles(djt, N, jt_blocks, jt_ntc, st));
   std::vector<int> active(N);
   for (int q = 0; q < N; ++q) active[q] = q;
   std::vector<int> sweeps_used(N, E->max_sweeps);
@@ -928,7 +941,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       for (int l = 0; l < lanes; ++l)
         if (lane_n[l] > 0 && round < lane_rounds[l])
           HIP_TRY(launch_jacobi_round(djt + lane_begin[l], lane_n[l], lane_pairs[l], round, E->inner_sweeps,
-                                      E->lane_stream[l]));
+                                      E->lane_stream[l], true));
     for (int l = 1; l < lanes; ++l) {
       HIP_TRY(hipEventRecord(E->lane_ev[l], E->lane_stream[l]));
       HIP_TRY(hipStreamWaitEvent(st, E->lane_ev[l], 0));
@@ -972,6 +985,9 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     }
     active.swap(next);
   }
+  // rows back from the tiles (the full task list: the device copy held the last active list)
+  HIP_TRY(hipMemcpyAsync(djt, hjt, N * sizeof(JacTask), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_from_tiles(djt, N, jt_blocks, jt_ntc, st));
   if (nlq > 0) {  // rows of the Jacobi result back to the column space of X
     RC_TRY(fork_lanes(lq_lanes));
     for (int l = 0; l < lq_lanes; ++l) {
@@ -983,7 +999,6 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     }
     RC_TRY(join_lanes(lq_lanes));
   }
-  // restore the full JacTask array (not needed further, kept for clarity of ownership)
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[2], st));
 
   // ---- singular values, truncation, factors ----
@@ -1288,6 +1303,9 @@ int run_kraus_waves(hipmps_engine E, const std::vector<hipmps_state_s*>& states)
     J.tol = s->fp32 ? E->tol_mixed : E->tol;
     J.gram32 = 0;
     J.pad = 0;
+    J.Yt = nullptr;  // row-major sweeps
+    J.ntc = 0;
+    J.pad2 = 0;
     ko += 4 * (size_t)h.n_ops;
   }
   char* dd = (char*)E->desc.p;

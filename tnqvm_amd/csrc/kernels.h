@@ -45,7 +45,21 @@ struct JacTask {
   double tol;        // relative off-diagonal below which a block pair counts as orthogonal
   int gram32;        // this sweep forms the Gram with FP32 MFMA (rotations and updates stay FP64)
   int pad;
+  double2* Yt;       // the same rows in tiles (jac_tile_offset), or null: the sweeps of the main path
+                     // run on Yt (launch_to_tiles before, launch_from_tiles after)
+  int ntc;           // 16-column tiles per 16-row block of Yt: ceil(c_tot / 16)
+  int pad2;
 };
+
+// Tiled row layout of the Jacobi sweeps: 16-row blocks, each a sequence of 16 x 16 tiles, each
+// tile 4 x 4 sub-tiles of 4 rows x 4 columns (256 B, row-major) in row-major order.  A Gram load
+// of 16 rows x 4 columns is then four 256 B segments (not sixteen 64 B ones 8 KB apart), and an
+// apply load of 4 rows x 16 columns one 1 KB segment.
+__host__ __device__ inline size_t jac_tile_offset(int row, int col, int ntc) {
+  const int i = row & 15;
+  return ((size_t)((row >> 4) * ntc + (col >> 4)) * 16 + (i >> 2) * 4 + ((col & 15) >> 2)) * 16 + (i & 3) * 4 +
+         (col & 3);
+}
 
 // Everything needed after convergence: singular values, truncation, factors.
 // Split after convergence.  Row k of the final Y is sigma_k xh_k^H (xh_k the k-th right
@@ -150,7 +164,11 @@ constexpr double JAC_FLOOR2 = 1e-30;
 // *jt[t].fro2 = ||theta'_t||_F^2 (= ||X||_F^2 = ||R2||_F^2), one workgroup per task, fixed order
 hipError_t launch_fro2(const SplitTask* d_split, const JacTask* d_jac, int ntasks, hipStream_t s);
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
-                               int inner_sweeps, hipStream_t s);
+                               int inner_sweeps, hipStream_t s, bool tiled = false);
+// Y (row-major, ldy) <-> Yt (tiled, ntc) for every task: rows [0, r_pad), columns [0, c_tot);
+// to_tiles writes zeros into the tile columns past c_tot.  max_blocks = largest r_pad / 16.
+hipError_t launch_to_tiles(const JacTask* d_tasks, int ntasks, int max_blocks, int max_ntc, hipStream_t s);
+hipError_t launch_from_tiles(const JacTask* d_tasks, int ntasks, int max_blocks, int max_ntc, hipStream_t s);
 // Householder LQ of every task's M (panel by panel, one launch per panel).
 hipError_t launch_lq(const LqTask* d_tasks, int ntasks, int max_rows, hipStream_t s);
 // The same LQ with look-ahead on two streams: on sa, the next panel's rows take the current block

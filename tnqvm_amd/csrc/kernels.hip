@@ -286,7 +286,7 @@ __device__ __forceinline__ void jacobi_rotation(double alpha, double beta, doubl
 //    to x+1 (31 -> 1) from one step to the next, except row 0.
 // BLK is the block size JB (a template parameter: the kernel's register allocation on
 // gfx950 comes out at 150 VGPRs, three waves per SIMD, in this form).
-template <int BLK>
+template <int BLK, bool TILED>
 __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, int ntasks, int round,
                                                          int inner_sweeps) {
   static_assert(BLK == 16 && JB == BLK && JP == 2 * BLK, "256 threads map onto 16 x 16 rotation-pair blocks");
@@ -307,12 +307,16 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   const size_t ld = T.ldy;
   const double tol = T.tol;
   const double floor2 = T.fro2 ? JAC_FLOOR2 * *T.fro2 : 0.0;
-  double2* Y = T.Y;
+  double2* Y = TILED ? T.Yt : T.Y;
+  const int ntc = T.ntc;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int l16 = lane & 15, k4 = lane >> 4;
-  const double2* ra = Y + (size_t)(ba * JB + l16) * ld;  // pair row l16 (block a)
-  const double2* rb = Y + (size_t)(bb * JB + l16) * ld;  // pair row 16 + l16 (block b)
+  // element (row, col) of the pair's rows; row-major: row * ld + col
+  auto at = [&](int row, int col) -> size_t {
+    return TILED ? jac_tile_offset(row, col, ntc) : (size_t)row * ld + col;
+  };
+  const int rowa = ba * JB + l16, rowb = bb * JB + l16;  // pair rows l16 and 16 + l16
 
   // ---- Gram tiles (0,0), (0,1), (1,1) over this wave's 4-column units u = wave, wave + 4, ... ----
   // FP64 MFMA, or in the early sweeps of large SVDs FP32 MFMA at twice the rate: the rotations
@@ -323,8 +327,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   auto ld_unit = [&](int u, double2& v0, double2& v1) {
     const int col = 4 * u + k4;
     if (u < nu && col < C) {
-      v0 = ra[col];
-      v1 = rb[col];
+      v0 = Y[at(rowa, col)];
+      v1 = Y[at(rowb, col)];
     } else {
       v0 = make_double2(0, 0);
       v1 = make_double2(0, 0);
@@ -464,8 +468,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
 #pragma unroll
     for (int kk = 0; kk < 8; ++kk) {
       const int row = kk * 4 + k4;
-      const double2* rp = Y + (size_t)(row < JB ? ba * JB + row : bb * JB + row - JB) * ld;
-      b[kk] = ok ? rp[col] : make_double2(0, 0);
+      b[kk] = ok ? Y[at(row < JB ? ba * JB + row : bb * JB + row - JB, col)] : make_double2(0, 0);
     }
   };
   ld_tile(wave, bq);
@@ -572,7 +575,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
       const double2 w0 = sW[kk * 4 + k4][l16];       // A[i][k] = W[k][i], rows 0..15
       const double2 w1 = sW[kk * 4 + k4][16 + l16];  // rows 16..31
       const int row = kk * 4 + k4;
-      bq[kk] = nok ? Y[(size_t)(row < JB ? ba * JB + row : bb * JB + row - JB) * ld + ncol] : make_double2(0, 0);
+      bq[kk] = nok ? Y[at(row < JB ? ba * JB + row : bb * JB + row - JB, ncol)] : make_double2(0, 0);
       const double dmc = b.y - b.x, cpd = b.x + b.y;
       e0 = mfma64(w0.x + w0.y, b.x, e0);
       e1 = mfma64(w1.x + w1.y, b.x, e1);
@@ -585,8 +588,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     if (col < CT) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        Y[(size_t)(ba * JB + 4 * v + k4) * ld + col] = make_double2(e0[v] - g0[v], e0[v] + f0[v]);
-        Y[(size_t)(bb * JB + 4 * v + k4) * ld + col] = make_double2(e1[v] - g1[v], e1[v] + f1[v]);
+        Y[at(ba * JB + 4 * v + k4, col)] = make_double2(e0[v] - g0[v], e0[v] + f0[v]);
+        Y[at(bb * JB + 4 * v + k4, col)] = make_double2(e1[v] - g1[v], e1[v] + f1[v]);
       }
     }
   }
@@ -615,9 +618,39 @@ hipError_t launch_fro2(const SplitTask* d_split, const JacTask* d_jac, int ntask
 }
 
 hipError_t launch_jacobi_round(const JacTask* d_tasks, int ntasks, int total_pairs, int round,
-                               int inner_sweeps, hipStream_t s) {
+                               int inner_sweeps, hipStream_t s, bool tiled) {
   if (total_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_jacobi_round<JB>, dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round, inner_sweeps);
+  if (tiled)
+    hipLaunchKernelGGL((k_jacobi_round<JB, true>), dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round,
+                       inner_sweeps);
+  else
+    hipLaunchKernelGGL((k_jacobi_round<JB, false>), dim3(total_pairs), dim3(256), 0, s, d_tasks, ntasks, round,
+                       inner_sweeps);
+  return hipGetLastError();
+}
+
+// One 16 x 16 tile per workgroup (blockIdx.x = 16-row block * max_ntc + tile column, blockIdx.y =
+// task); thread (i, j) = (tid / 16, tid % 16) moves element (16 b + i, 16 t + j).
+template <bool TO>
+__global__ __launch_bounds__(256) void k_tiles(const JacTask* tasks, int max_ntc) {
+  const JacTask& T = tasks[blockIdx.y];
+  const int b = blockIdx.x / max_ntc, t = blockIdx.x % max_ntc;
+  if (b >= T.r_pad / 16 || t >= T.ntc) return;
+  const int row = 16 * b + (threadIdx.x >> 4), col = 16 * t + (threadIdx.x & 15);
+  const size_t o = jac_tile_offset(row, col, T.ntc);
+  if (TO) T.Yt[o] = col < T.c_tot ? T.Y[(size_t)row * T.ldy + col] : make_double2(0, 0);
+  else if (col < T.c_tot) T.Y[(size_t)row * T.ldy + col] = T.Yt[o];
+}
+
+hipError_t launch_to_tiles(const JacTask* d_tasks, int ntasks, int max_blocks, int max_ntc, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tiles<true>, dim3(max_blocks * max_ntc, ntasks), dim3(256), 0, s, d_tasks, max_ntc);
+  return hipGetLastError();
+}
+
+hipError_t launch_from_tiles(const JacTask* d_tasks, int ntasks, int max_blocks, int max_ntc, hipStream_t s) {
+  if (ntasks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tiles<false>, dim3(max_blocks * max_ntc, ntasks), dim3(256), 0, s, d_tasks, max_ntc);
   return hipGetLastError();
 }
 
