@@ -323,8 +323,7 @@ __global__ __launch_bounds__(512) void k_lq_factor_lds(const LqTask* tasks, int 
   __shared__ double2 sG[LQB][LQB + 1];
   __shared__ double2 sT[LQB][LQB + 1];
   __shared__ double2 s_w0[LQB];
-  __shared__ double2 s_par[2];  // beta, w0
-  __shared__ double s_scale;
+  __shared__ double s_scale[LQB];
   const LqTask& T = tasks[blockIdx.x];
   const int k0 = p * LQB;
   if (k0 >= T.rows) return;
@@ -338,46 +337,53 @@ __global__ __launch_bounds__(512) void k_lq_factor_lds(const LqTask* tasks, int 
     sy[i][c] = i < nr ? M[(size_t)(k0 + i) * ld + k0 + c] : make_double2(0, 0);
   }
   __syncthreads();
-  for (int r = 0; r < nr; ++r) {
-    if (wave == 0) {
-      double s = 0.0;
-      for (int c = r + 1 + lane; c < W; c += 64) s += cabs2(sy[r][c]);
-      s = wave_sum(s);
-      if (lane == 0) {
-        const double2 a = sy[r][r];
-        const double nrm = sqrt(cabs2(a) + s);
-        double2 beta = make_double2(0, 0), w0 = make_double2(0, 0);
-        double scale = 0.0;
-        if (nrm > 0.0) {
-          const double aa = sqrt(cabs2(a));
-          const double2 ph = aa > 0.0 ? cscale(a, 1.0 / aa) : make_double2(1.0, 0.0);
-          beta = cscale(ph, -nrm);  // -e^{i arg a} |x|: no cancellation in a - beta
-          const double2 v0 = make_double2(a.x - beta.x, a.y - beta.y);
-          scale = 1.0 / sqrt(cabs2(v0) + s);
-          w0 = cscale(v0, scale);
-        }
-        s_par[0] = beta;
-        s_par[1] = w0;
-        s_scale = scale;
+  // Reflector of row k (wave 0 only): w = (w0 at column k, scale * row k beyond), beta onto the
+  // diagonal.  Row k's tail stays unscaled in LDS: every use multiplies by s_scale[k], the same
+  // single rounding as storing it scaled.
+  auto reflector = [&](int k) {
+    double s = 0.0;
+    for (int c = k + 1 + lane; c < W; c += 64) s += cabs2(sy[k][c]);
+    s = wave_sum(s);
+    if (lane == 0) {
+      const double2 a = sy[k][k];
+      const double nrm = sqrt(cabs2(a) + s);
+      double2 beta = make_double2(0, 0), w0 = make_double2(0, 0);
+      double scale = 0.0;
+      if (nrm > 0.0) {
+        const double aa = sqrt(cabs2(a));
+        const double2 ph = aa > 0.0 ? cscale(a, 1.0 / aa) : make_double2(1.0, 0.0);
+        beta = cscale(ph, -nrm);  // -e^{i arg a} |x|: no cancellation in a - beta
+        const double2 v0 = make_double2(a.x - beta.x, a.y - beta.y);
+        scale = 1.0 / sqrt(cabs2(v0) + s);
+        w0 = cscale(v0, scale);
       }
+      sy[k][k] = beta;
+      s_w0[k] = w0;
+      s_scale[k] = scale;
     }
-    __syncthreads();
-    const double2 w0 = s_par[1];
-    const double scale = s_scale;
-    // rows below: y -= 2 <y, w> w, w = (w0 at c = r, scale * row r beyond)
+  };
+  if (wave == 0 && nr > 0) reflector(0);
+  __syncthreads();
+  // One barrier per reflector: wave w updates rows i = r + 1 + w (mod 8); wave 0 owns row r + 1,
+  // the next pivot, and forms its reflector as soon as that row is updated.
+  for (int r = 0; r < nr; ++r) {
+    const double2 w0 = s_w0[r];
+    const double scale = s_scale[r];
     for (int i = r + 1 + wave; i < nr; i += NW) {
       double2 d = lane == 0 ? cfmac(w0, sy[i][r], make_double2(0, 0)) : make_double2(0, 0);
       for (int c = r + 1 + lane; c < W; c += 64) d = cfmac(cscale(sy[r][c], scale), sy[i][c], d);
-      d = wave_sum2(d);
+      d = wave_sum2(d);  // <y, w> = sum y_l conj(w_l)
       const double2 m2d = cscale(d, -2.0);
       if (lane == 0) sy[i][r] = cfma(m2d, w0, sy[i][r]);
       for (int c = r + 1 + lane; c < W; c += 64) sy[i][c] = cfma(m2d, cscale(sy[r][c], scale), sy[i][c]);
-    }
-    __syncthreads();  // every read of row r done
-    for (int c = r + 1 + tid; c < W; c += NT) sy[r][c] = cscale(sy[r][c], scale);
-    if (tid == 0) {
-      sy[r][r] = s_par[0];
-      s_w0[r] = w0;
+      if (i == r + 1) {
+        // the row's new values were written by other lanes of this wave: order them before the
+        // reflector's reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        reflector(i);
+      }
     }
     __syncthreads();
   }
@@ -387,32 +393,39 @@ __global__ __launch_bounds__(512) void k_lq_factor_lds(const LqTask* tasks, int 
     if (j <= i) continue;
     double2 d = make_double2(0, 0);
     if (j < nr) {
-      if (lane == 0) d = cfmac(s_w0[j], sy[i][j], d);
-      for (int c = j + 1 + lane; c < W; c += 64) d = cfmac(sy[j][c], sy[i][c], d);
+      const double si = s_scale[i], sj = s_scale[j];
+      if (lane == 0) d = cfmac(s_w0[j], cscale(sy[i][j], si), d);
+      for (int c = j + 1 + lane; c < W; c += 64) d = cfmac(cscale(sy[j][c], sj), cscale(sy[i][c], si), d);
     }
     d = wave_sum2(d);
     if (lane == 0) sG[i][j] = d;
   }
   __syncthreads();
-  for (int j = 0; j < LQB; ++j) {
-    if (tid < LQB) {
-      double2 t = make_double2(0, 0);
-      if (tid < j) {
-        for (int m = tid; m < j; ++m) t = cfma(sT[tid][m], sG[m][j], t);
-        t = cscale(t, -2.0);
-      } else if (tid == j) {
-        t = make_double2(k0 + j < k1 ? 2.0 : 0.0, 0.0);
+  // T column by column in wave 0 (lanes 0..15), ordered within the wave
+  if (wave == 0) {
+    for (int j = 0; j < LQB; ++j) {
+      if (lane < LQB) {
+        double2 t = make_double2(0, 0);
+        if (lane < j) {
+          for (int m = lane; m < j; ++m) t = cfma(sT[lane][m], sG[m][j], t);
+          t = cscale(t, -2.0);
+        } else if (lane == j) {
+          t = make_double2(k0 + j < k1 ? 2.0 : 0.0, 0.0);
+        }
+        sT[lane][j] = t;
       }
-      sT[tid][j] = t;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __syncthreads();
   }
+  __syncthreads();
   double2* Tp = T.T + (size_t)p * LQB * LQB;
   for (int e = tid; e < LQB * LQB; e += NT) Tp[e] = sT[e / LQB][e % LQB];
   if (tid < nr) T.W0[k0 + tid] = s_w0[tid];
   for (int e = tid; e < nr * W; e += NT) {
     const int i = e / W, c = e % W;
-    M[(size_t)(k0 + i) * ld + k0 + c] = sy[i][c];
+    M[(size_t)(k0 + i) * ld + k0 + c] = c > i ? cscale(sy[i][c], s_scale[i]) : sy[i][c];
   }
 }
 
