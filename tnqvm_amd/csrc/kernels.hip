@@ -269,6 +269,29 @@ __device__ __forceinline__ void jacobi_rotation(double alpha, double beta, doubl
   se = make_double2(c * tau.x, c * tau.y);
 }
 
+// Thread -> 2x2 block of rotation pairs (bk, bl) of the inner sweep, packed bk << 4 | bl.  The 16
+// blocks that form the next step's rotations -- (0, 1), (j + 1, j - 1) for 1 <= j <= 14, (15, 14)
+// -- go to threads 0..15 (wave 0), so that the other three waves skip that code; the remaining
+// 240 blocks follow in row order.
+struct InnerBlockMap {
+  unsigned char v[256];
+};
+constexpr bool inner_owner(int bk, int bl) {
+  return (bk == 0 && bl == 1) || (bk >= 2 && bl == bk - 2) || (bk == 15 && bl == 14);
+}
+constexpr InnerBlockMap make_inner_block_map() {
+  InnerBlockMap m{};
+  m.v[0] = (unsigned char)(0 << 4 | 1);
+  for (int j = 1; j <= 14; ++j) m.v[j] = (unsigned char)((j + 1) << 4 | (j - 1));
+  m.v[15] = (unsigned char)(15 << 4 | 14);
+  int n = 16;
+  for (int bk = 0; bk < 16; ++bk)
+    for (int bl = 0; bl < 16; ++bl)
+      if (!inner_owner(bk, bl)) m.v[n++] = (unsigned char)(bk << 4 | bl);
+  return m;
+}
+__constant__ InnerBlockMap c_inner_blocks = make_inner_block_map();
+
 // Design notes (gfx950):
 //  * Gram and apply stream their MFMA operands straight from memory into registers; no
 //    LDS staging and no barriers in either loop.  The Gram is Hermitian, so only tiles
@@ -489,7 +512,8 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   // writes its rotated 2x2 blocks where the next step reads them (a row moves from slot x to
   // x-1, slot 1 to JP-1, slot 0 stays), so every G address is fixed per thread; after JP-1
   // steps the slots are back in row order.  W stays in row order (single buffer).
-  const int bk = tid >> 4, bl = tid & 15;  // G block of rotation pairs bk, bl
+  const int blk = c_inner_blocks.v[tid];
+  const int bk = blk >> 4, bl = blk & 15;  // G block of rotation pairs bk, bl
   const int wi = tid >> 3, wl = tid & 7;   // W row wi, rotation pairs wl and wl + 8
   auto nxt = [](int x) { return x == JP - 1 ? 1 : x + 1; };  // row index one step later
   auto prv = [](int x) { return x == 0 ? 0 : (x == 1 ? JP - 1 : x - 1); };  // slot one step later
@@ -515,8 +539,6 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     const double c2 = s_c[buf][wl], c3 = s_c[buf][wl + 8];
     const double2 se2 = s_se[buf][wl], se3 = s_se[buf][wl + 8];
     const double2 w0p = sW[wi][pw0], w0q = sW[wi][qw0], w1p = sW[wi][pw1], w1q = sW[wi][qw1];
-    const double ak = gA[pk][pk].x, bk2 = gA[qk][qk].x, al2 = gA[pl][pl].x, bl2 = gA[ql][ql].x;
-    const double2 gk = gA[pk][qk], gl = gA[pl][ql];
     // ---- block B' = Jk^H B Jl,  J = [[c, se], [-conj(se), c]] ----
     const double2 nsel = make_double2(-sel.x, sel.y);
     const double2 m00 = cfma(b01, nsel, cscale(b00, cl)), m01 = cfma(b00, sel, cscale(b01, cl));
@@ -529,17 +551,21 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     const double2 v0q = cfma(w0p, se2, cscale(w0q, c2));
     const double2 v1p = cfma(w1q, make_double2(-se3.x, se3.y), cscale(w1p, c3));
     const double2 v1q = cfma(w1p, se3, cscale(w1q, c3));
-    // ---- the next step's rotation (kept by its owner only) ----
-    const double rk = 2.0 * ck * (gk.x * sek.x + gk.y * sek.y);
-    const double ek = cabs2(sek), c2k = ck * ck;
-    const double alpha = rra ? (ek * ak + c2k * bk2 + rk) : (c2k * ak + ek * bk2 - rk);
-    const double rl = 2.0 * cl * (gl.x * sel.x + gl.y * sel.y);
-    const double el = cabs2(sel), c2l = cl * cl;
-    const double beta = rrb ? (el * al2 + c2l * bl2 + rl) : (c2l * al2 + el * bl2 - rl);
-    const double2 gam = rra ? (rrb ? n11 : n10) : (rrb ? n01 : n00);
-    double cn;
-    double2 sen;
-    jacobi_rotation(alpha, beta, gam, itol, floor2, cn, sen);
+    // ---- the next step's rotation: wave 0 (threads 0..15 own one each) ----
+    double cn = 1.0;
+    double2 sen = make_double2(0.0, 0.0);
+    if (wave == 0) {
+      const double ak = gA[pk][pk].x, bk2 = gA[qk][qk].x, al2 = gA[pl][pl].x, bl2 = gA[ql][ql].x;
+      const double2 gk = gA[pk][qk], gl = gA[pl][ql];
+      const double rk = 2.0 * ck * (gk.x * sek.x + gk.y * sek.y);
+      const double ek = cabs2(sek), c2k = ck * ck;
+      const double alpha = rra ? (ek * ak + c2k * bk2 + rk) : (c2k * ak + ek * bk2 - rk);
+      const double rl = 2.0 * cl * (gl.x * sel.x + gl.y * sel.y);
+      const double el = cabs2(sel), c2l = cl * cl;
+      const double beta = rrb ? (el * al2 + c2l * bl2 + rl) : (c2l * al2 + el * bl2 - rl);
+      const double2 gam = rra ? (rrb ? n11 : n10) : (rrb ? n01 : n00);
+      jacobi_rotation(alpha, beta, gam, itol, floor2, cn, sen);
+    }
     // ---- all LDS writes of the step ----
     gB[dpk][dpl] = n00;
     gB[dpk][dql] = n01;
@@ -549,8 +575,10 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
     sW[wi][qw0] = v0q;
     sW[wi][pw1] = v1p;
     sW[wi][qw1] = v1q;
-    s_c[buf ^ 1][slot] = cn;
-    s_se[buf ^ 1][slot] = sen;
+    if (wave == 0) {
+      s_c[buf ^ 1][slot] = cn;
+      s_se[buf ^ 1][slot] = sen;
+    }
     __syncthreads();
     double2 (*tmp)[JP + 1] = gA;
     gA = gB;
