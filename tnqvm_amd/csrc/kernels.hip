@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
   __shared__ double2 sW[JP][JP + 1];
   __shared__ double s_c[2][JP / 2 + 1];  // rotations of the current / next step;
   __shared__ double2 s_se[2][JP / 2 + 1];  // entry JP/2 takes the writes of non-owners
-  __shared__ double s_red[256];
+  __shared__ double s_red[4];  // per-wave maxima of the convergence test
 
   const int t = find_task(tasks, ntasks, (int)blockIdx.x, [](const JacTask& x) { return x.pair_begin; });
   const JacTask& T = tasks[t];
@@ -472,15 +472,13 @@ __global__ __launch_bounds__(256, 2) void k_jacobi_round(const JacTask* tasks, i
       double aii = sG[i][i].x, ajj = sG[j][j].x;
       if (aii > floor2 && ajj > floor2) m = fmax(m, cabs2(sG[i][j]) / (aii * ajj));
     }
-    s_red[tid] = m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane == 0) s_red[wave] = m;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
-      __syncthreads();
-    }
   }
   if (tid == 0) atomicAdd(T.visited, 1);
-  if (s_red[0] <= tol * tol) return;
+  if (fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3])) <= tol * tol) return;
 
   // ---- apply prefetch: this wave's first column tile, all 32 rows ----
   const int nt = (CT + 15) >> 4;
