@@ -886,6 +886,10 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
   for (int q = 0; q < N; ++q) active[q] = q;
   std::vector<int> sweeps_used(N, E->max_sweeps);
   std::vector<char> converged(N, 0);
+  // pairs each item rotated in the previous sweep: within a lane, the busiest items' block pairs
+  // come first in the launch, so that the workgroups starting last (in the launch's tail) are
+  // mostly the short ones of nearly converged items, which only form their Gram
+  std::vector<long long> prev_rot(N, LLONG_MAX);
   int* hcounts = (int*)((char*)E->hdesc.p + desc_bytes);
   for (int sweep = 0; sweep < E->max_sweeps && !active.empty(); ++sweep) {
     // active task list with pair prefix; with several lanes, the items are dealt to lists
@@ -910,6 +914,9 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
       lane_items[l].push_back(q);
       lane_load[l] += (long long)(hjt[q].nb / 2) * hjt[q].c * (hjt[q].nb - 1);
     }
+    for (int l = 0; l < lanes; ++l)
+      std::stable_sort(lane_items[l].begin(), lane_items[l].end(),
+                       [&](int a, int b) { return prev_rot[a] > prev_rot[b]; });
     int lane_begin[ML] = {0, 0, 0, 0}, lane_n[ML] = {0, 0, 0, 0}, lane_pairs[ML] = {0, 0, 0, 0},
         lane_rounds[ML] = {0, 0, 0, 0};
     {
@@ -971,6 +978,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     for (int a = 0; a < nact; ++a) {
       int q = active[a];
       long long rot = hcounts[2 * q], vis = hcounts[2 * q + 1];
+      prev_rot[q] = rot;
       E->prof.jacobi_pairs_rotated += rot;
       E->prof.jacobi_pairs_visited += vis;
       const double c = items[q].lq ? items[q].r : items[q].c;  // columns the Jacobi works on
