@@ -225,3 +225,32 @@ def test_serve_admission(eng):
             assert r["max_bond"] == w["max_bond"]
     finally:
         e2.close()
+
+
+def test_descriptor_upload_held_back(eng, monkeypatch):
+    """A wave's descriptor upload held back on the device (HIPMPS_TEST_STAGE_DELAY_US queues a
+    2 ms sleeping wave ahead of it, as a stream waiting behind other work on a shared hardware
+    queue would): the host edits the staged Jacobi descriptors of LQ-preconditioned items only
+    after the copy has read them, so every batch result equals the undelayed engine's, bit for
+    bit, and a state run alone on the delayed engine agrees to 1e-12."""
+    circs = [{"n": 12, "gates": C.haar_brickwork(12, 8, seed=0x7E5A0B00 + k), "chi": 32} for k in range(3)]
+    circs.append({"n": 14, "gates": C.haar_brickwork(14, 10, seed=0x7E5A0108), "chi": 64})
+    jobs = _jobs(circs, "mixed") + _jobs(circs, "fp64")
+    want = eng.run_batch(jobs)
+    monkeypatch.setenv("HIPMPS_TEST_STAGE_DELAY_US", "2000")
+    e2 = mps.Engine()
+    monkeypatch.delenv("HIPMPS_TEST_STAGE_DELAY_US")
+    try:
+        say("held-back upload", len(jobs), "circuits")
+        got = e2.run_batch(jobs)
+        s = mps.MpsState(e2, 14, cutoff=1e-10, max_bond=64).run(circs[3]["gates"])
+        z_state = s.expval_z_all()
+        s.close()
+    finally:
+        e2.close()
+    for g, w in zip(got, want):
+        assert g["status"] == 0 and w["status"] == 0
+        assert np.array_equal(g["expval_z"], w["expval_z"])
+        assert np.array_equal(g["expval_zz"], w["expval_zz"])
+        assert g["max_bond"] == w["max_bond"]
+    assert np.abs(z_state - want[len(circs) + 3]["expval_z"]).max() < 1e-12

@@ -51,6 +51,14 @@ using namespace hipmps;
 // per device when an engine is created on it (engine_scratch_setup).
 static __device__ __align__(256) char scratch[HIPMPS_NOALLOC_SCRATCH_BYTES];
 
+// Test hook (HIPMPS_TEST_STAGE_DELAY_US): one wave that sleeps for `ticks` of the constant-rate
+// wall clock, queued on a wave's stream ahead of its descriptor upload, so that the upload runs
+// late, as it does when the stream waits behind other work on a shared hardware queue.
+__global__ void k_stage_delay(long long ticks) {
+  const long long t0 = wall_clock64();
+  while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
+}
+
 namespace {
 
 thread_local std::string t_err;
@@ -188,6 +196,10 @@ struct hipmps_engine_s {
   // on another) for rows of at most LQ_REG_MAX_COLS columns; HIPMPS_LQ_LOOKAHEAD=0 turns it off
   int lq_lookahead = 1;
   hipEvent_t lq_ev[2 * MAX_LANES] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // completion of a wave's descriptor upload: the host edits the staged Jacobi descriptors
+  // after enqueuing that copy, and the copy may still wait behind earlier work on the stream
+  hipEvent_t stage_ev = nullptr;
+  long long stage_delay_ticks = 0;  // test hook: the upload held back this long (wall-clock ticks)
   DevBuf ws;     // per-wave arena
   DevBuf desc;   // task descriptors
   DevBuf small;  // counters / K / stats
@@ -783,7 +795,20 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     granges.push_back({q, e, tiles});
     q = e;
   }
+  if (E->stage_delay_ticks > 0) {
+    hipLaunchKernelGGL(k_stage_delay, dim3(1), dim3(64), 0, st, E->stage_delay_ticks);
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipMemcpyAsync(dd, hd, desc_bytes, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(E->stage_ev, st));
+  // the copy reads the staging buffer when the stream reaches it; the host edits the Jacobi
+  // descriptors there (LQ items switch to R2) only once it has been read
+  bool staged = false;
+  auto wait_staged = [&]() -> int {
+    if (!staged) HIP_TRY(hipEventSynchronize(E->stage_ev));
+    staged = true;
+    return 0;
+  };
 
   if (E->timing) HIP_TRY(hipEventRecord(E->ev[0], st));
   // purified left sites: Kraus leg next to the left bond, [l][p][k][m] -> [l][k][p][m]
@@ -859,6 +884,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
     }
     RC_TRY(join_lanes(lq_lanes));
     if (any_cut) HIP_TRY(launch_rank_cut(dtr + N, dcut, nlq, st));
+    RC_TRY(wait_staged());
     for (int q = 0; q < N; ++q) {
       auto& it = items[q];
       if (!it.lq) continue;
@@ -873,6 +899,7 @@ int run_wave(hipmps_engine E, std::vector<WaveItem>& items) {
 
   // ---- Jacobi sweeps, on a tiled copy of every Jacobi matrix ----
   int jt_blocks = 0, jt_ntc = 0;
+  RC_TRY(wait_staged());
   for (int q = 0; q < N; ++q) {
     JacTask& J = hjt[q];
     J.Yt = (double2*)(ws + items[q].off_jt);
@@ -2294,6 +2321,12 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
   for (int l = 1; l < hipmps_engine_s::MAX_LANES; ++l)
     ok = ok && hipStreamCreateWithFlags(&e->lane_stream[l], hipStreamNonBlocking) == hipSuccess;
   for (auto& ev : e->lane_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+  if (hipEventCreateWithFlags(&e->stage_ev, hipEventDisableTiming) != hipSuccess) {
+    e->stage_ev = nullptr;
+    (void)hipGetLastError();
+    (void)hipmps_engine_destroy(e);
+    return fail(HIPMPS_E_DEVICE, "event creation failed");
+  }
   bool lq_ok = true;
   for (auto& ev : e->lq_ev) lq_ok = lq_ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
   if (!lq_ok) {
@@ -2301,6 +2334,14 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
     e->lq_lookahead = 0;
   }
   if (const char* v = getenv("HIPMPS_LQ_LOOKAHEAD")) e->lq_lookahead = atoi(v) != 0;
+  if (const char* v = getenv("HIPMPS_TEST_STAGE_DELAY_US")) {
+    int khz = 0;  // wall-clock rate
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device) != hipSuccess || khz <= 0) {
+      (void)hipGetLastError();
+      khz = 100000;
+    }
+    e->stage_delay_ticks = (long long)std::max(0, std::min(100000, atoi(v))) * khz / 1000;
+  }
   if (!ok) {
     (void)hipGetLastError();
     e->jacobi_lanes = 1;
@@ -2346,6 +2387,7 @@ int hipmps_engine_destroy(hipmps_engine e) {
     if (ev) (void)hipEventDestroy(ev);
   for (auto& ev : e->lq_ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (e->stage_ev) (void)hipEventDestroy(e->stage_ev);
   for (int l = 1; l < hipmps_engine_s::MAX_LANES; ++l)
     if (e->lane_stream[l]) {
       (void)hipStreamSynchronize(e->lane_stream[l]);
