@@ -258,7 +258,8 @@ int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits,
  * queued circuit that fails later fails alone, with its code in result->status. */
 int hipmps_serve_submit(hipmps_engine e, const hipmps_circuit* c, hipmps_result* result,
                         uint64_t* ticket);
-/* advance every in-flight circuit by one wave; circuits on the background lane (see
+/* advance every in-flight circuit by one wave (the two wave groups concurrently, see
+ * hipmps_tuning.wave_groups); circuits on the background lane (see
  * hipmps_tuning.big_lane_min_chi) advance on a worker thread of the engine meanwhile */
 int hipmps_serve_step(hipmps_engine e, int* n_done);
 int hipmps_serve_poll(hipmps_engine e, uint64_t ticket, int* done);
@@ -310,6 +311,11 @@ typedef struct {
   int jacobi_max_sweeps; /* sweep cap of every Jacobi SVD (1..1000), after which the SVD counts as
                             non-converged (HIPMPS_E_CONVERGENCE, hipmps_stats_t.svds_nonconverged);
                             default: hipmps_engine_opts.jacobi_max_sweeps */
+  int wave_groups;      /* batches and serving: circuits are split over this many groups (1 or 2,
+                           default 2), each with an engine of its own (streams, arena); the groups'
+                           waves advance concurrently, the second on a worker thread, so that one
+                           group's LQ panels overlap the other's Jacobi rounds.  A circuit's results
+                           do not depend on its group.  hipmps_engine_profile sums both groups */
 } hipmps_tuning;
 int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out);
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t);

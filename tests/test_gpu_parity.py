@@ -361,12 +361,14 @@ def test_empty_and_one_qubit_circuits(eng):
 
 
 def test_tuning_roundtrip_and_results(eng):
-    """Lanes only change the schedule (identical results, bit for bit); FP32-Gram sweeps only
-    the mixed path's rounding; bad values are refused."""
+    """Lanes and wave groups only change the schedule (identical results, bit for bit); FP32-Gram
+    sweeps only the mixed path's rounding; bad values are refused."""
     say("tuning")
     base = eng.tuning()
     with pytest.raises(Exception):
         eng.set_tuning(jacobi_lanes=0)
+    with pytest.raises(Exception):
+        eng.set_tuning(wave_groups=3)
     circs = [Cc.haar_brickwork(12, 8, seed=0x7E5A0B00 + k) for k in range(6)]
 
     def run(precision):
@@ -385,5 +387,9 @@ def test_tuning_roundtrip_and_results(eng):
         c64, cmx = run("fp64"), run("mixed")
         assert np.array_equal(a64, c64)  # FP64 states never take the FP32 Gram
         assert np.abs(cmx - amx).max() < 1e-6
+        for groups in (1, 2):
+            eng.set_tuning(wave_groups=groups)
+            assert eng.tuning()["wave_groups"] == groups
+            assert np.array_equal(run("fp64"), c64) and np.array_equal(run("mixed"), cmx)
     finally:
         eng.set_tuning(**base)

@@ -69,7 +69,7 @@ class Gate(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("jacobi_lanes", ctypes.c_int), ("lane_min_pairs", ctypes.c_int), ("gram32_sweeps", ctypes.c_int),
                 ("lq_min_rank", ctypes.c_int), ("big_lane_min_chi", ctypes.c_int),
-                ("rdm_block_elems", ctypes.c_int), ("jacobi_max_sweeps", ctypes.c_int)]
+                ("rdm_block_elems", ctypes.c_int), ("jacobi_max_sweeps", ctypes.c_int), ("wave_groups", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

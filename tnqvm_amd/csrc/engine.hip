@@ -234,6 +234,19 @@ struct hipmps_engine_s {
   bool big_stop = false;
   int big_rc = 0;
   std::string big_err;
+  // wave groups of batches and serving: the circuits are split over wave_groups engines (this
+  // one and grp), whose waves advance concurrently, grp's on a worker thread, so that one
+  // group's latency-bound phases (the LQ panels) overlap the other's Jacobi rounds.  A circuit
+  // stays on the engine it started on; each engine alone touches its circuits.
+  int wave_groups = 2;
+  hipmps_engine_s* grp = nullptr;
+  std::thread grp_worker;
+  std::mutex grp_mu;
+  std::condition_variable grp_cv;
+  std::vector<PendingCircuit*>* grp_job = nullptr;  // posted by the step, cleared by the worker
+  bool grp_stop = false;
+  int grp_rc = 0;
+  std::string grp_err;
 };
 
 struct hipmps_state_s {
@@ -284,6 +297,7 @@ struct PendingCircuit {
   bool finished = false;
   std::atomic<bool> done{false};
   bool big = false;  // runs on the background lane
+  int grp = 0;       // wave group (engine) it runs on
   hipmps_circuit c{};
   std::vector<hipmps_gate> gates;
   std::vector<int> zz;
@@ -2342,6 +2356,7 @@ int hipmps_engine_create(const hipmps_engine_opts* opts, hipmps_engine* out) {
     }
     e->stage_delay_ticks = (long long)std::max(0, std::min(100000, atoi(v))) * khz / 1000;
   }
+  if (const char* v = getenv("HIPMPS_WAVE_GROUPS")) e->wave_groups = std::max(1, std::min(2, atoi(v)));
   if (!ok) {
     (void)hipGetLastError();
     e->jacobi_lanes = 1;
@@ -2364,6 +2379,14 @@ int hipmps_engine_destroy(hipmps_engine e) {
     }
     e->big_cv.notify_all();
     if (e->big_worker.joinable()) e->big_worker.join();
+  }
+  if (e->grp) {  // the group worker is idle between steps; stop it before the circuits are freed
+    {
+      std::lock_guard<std::mutex> lk(e->grp_mu);
+      e->grp_stop = true;
+    }
+    e->grp_cv.notify_all();
+    if (e->grp_worker.joinable()) e->grp_worker.join();
   }
   (void)hipSetDevice(e->device);
   (void)hipStreamSynchronize(e->stream);
@@ -2395,6 +2418,7 @@ int hipmps_engine_destroy(hipmps_engine e) {
     }
   (void)hipStreamDestroy(e->stream);
   if (e->big) hipmps_engine_destroy(e->big);
+  if (e->grp) hipmps_engine_destroy(e->grp);
   delete e;
   return 0;
 }
@@ -2403,6 +2427,7 @@ int hipmps_engine_set_lq_precondition(hipmps_engine e, int min_rank) {
   if (!e) return fail(HIPMPS_E_ARG, "null engine");
   if (min_rank < 0) return fail(HIPMPS_E_ARG, "min_rank must be >= 0");
   e->lq_min_rank = min_rank;
+  if (e->grp) e->grp->lq_min_rank = min_rank;
   return 0;
 }
 
@@ -2415,6 +2440,7 @@ int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out) {
   out->big_lane_min_chi = e->big_min_chi;
   out->rdm_block_elems = e->rdm_block_elems;
   out->jacobi_max_sweeps = e->max_sweeps;
+  out->wave_groups = e->wave_groups;
   return 0;
 }
 
@@ -2422,7 +2448,7 @@ int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   if (!e || !t) return fail(HIPMPS_E_ARG, "null argument");
   if (t->jacobi_lanes < 1 || t->jacobi_lanes > hipmps_engine_s::MAX_LANES || t->lane_min_pairs < 0 ||
       t->gram32_sweeps < 0 || t->lq_min_rank < 0 || t->big_lane_min_chi < 0 || t->rdm_block_elems < 1 ||
-      t->jacobi_max_sweeps < 1 || t->jacobi_max_sweeps > 1000)
+      t->jacobi_max_sweeps < 1 || t->jacobi_max_sweeps > 1000 || t->wave_groups < 1 || t->wave_groups > 2)
     return fail(HIPMPS_E_ARG, "tuning values out of range");
   std::lock_guard<std::mutex> lock(e->mu);
   int made = 1;  // lanes whose stream exists
@@ -2434,19 +2460,52 @@ int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t) {
   e->big_min_chi = t->big_lane_min_chi;
   e->rdm_block_elems = t->rdm_block_elems;
   e->max_sweeps = t->jacobi_max_sweeps;
+  e->wave_groups = t->wave_groups;
+  if (e->grp) {  // the group engine follows, with no lane or groups of its own
+    hipmps_tuning g = *t;
+    g.big_lane_min_chi = 0;
+    g.wave_groups = 1;
+    const int rc = hipmps_engine_set_tuning(e->grp, &g);
+    if (rc) return rc;
+  }
   return 0;
 }
 
 int hipmps_engine_set_timing(hipmps_engine e, int enabled) {
   if (!e) return fail(HIPMPS_E_ARG, "null engine");
   e->timing = enabled != 0;
+  if (e->grp) e->grp->timing = e->timing;
   return 0;
 }
 
 int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset) {
   if (!e) return fail(HIPMPS_E_ARG, "null engine");
-  if (out) *out = e->prof;
-  if (reset) e->prof = hipmps_profile_t{};
+  if (out) {
+    *out = e->prof;
+    if (e->grp) {  // both wave groups; their phase times add up although the groups overlap in time
+      const hipmps_profile_t& g = e->grp->prof;
+      out->ms_theta += g.ms_theta;
+      out->ms_jacobi += g.ms_jacobi;
+      out->ms_split += g.ms_split;
+      out->ms_other += g.ms_other;
+      out->ms_jacobi_rounds += g.ms_jacobi_rounds;
+      out->jacobi_launches += g.jacobi_launches;
+      out->jacobi_pairs_rotated += g.jacobi_pairs_rotated;
+      out->jacobi_pairs_visited += g.jacobi_pairs_visited;
+      out->jacobi_flops += g.jacobi_flops;
+      out->waves += g.waves;
+      out->wave_tasks += g.wave_tasks;
+      out->jacobi_sweeps += g.jacobi_sweeps;
+      out->svds_large += g.svds_large;
+      out->sweeps_large += g.sweeps_large;
+      out->ms_lq += g.ms_lq;
+      for (int k = 0; k < 14; ++k) out->applies_by_rank[k] += g.applies_by_rank[k];
+    }
+  }
+  if (reset) {
+    e->prof = hipmps_profile_t{};
+    if (e->grp) e->grp->prof = hipmps_profile_t{};
+  }
   return 0;
 }
 
@@ -3088,11 +3147,99 @@ int ensure_big_lane(hipmps_engine E) {
   RC_TRY(hipmps_engine_create(&o, &B));
   hipmps_tuning t{};
   (void)hipmps_engine_get_tuning(E, &t);
+  t.big_lane_min_chi = 0;
+  t.wave_groups = 1;  // the worker advances the lane's circuits with advance_all, in one group
   (void)hipmps_engine_set_tuning(B, &t);
-  B->big_min_chi = 0;
   E->big = B;
   E->big_stop = false;
   E->big_worker = std::thread(big_lane_loop, E);
+  return 0;
+}
+
+// The second wave group's worker: run advance_all on the group engine for every posted job.
+void group_loop(hipmps_engine E) {
+  hipmps_engine B = E->grp;
+  (void)hipSetDevice(B->device);
+  for (;;) {
+    std::vector<PendingCircuit*>* job = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(E->grp_mu);
+      E->grp_cv.wait(lk, [&] { return E->grp_stop || E->grp_job != nullptr; });
+      if (E->grp_stop) return;
+      job = E->grp_job;
+    }
+    const int rc = advance_all(B, *job);
+    const std::string err = rc ? hipmps_last_error() : "";
+    {
+      std::lock_guard<std::mutex> lk(E->grp_mu);
+      E->grp_rc = rc;
+      E->grp_err = err;
+      E->grp_job = nullptr;
+    }
+    E->grp_cv.notify_all();
+  }
+}
+
+int ensure_group_engine(hipmps_engine E) {
+  if (E->grp) return 0;
+  hipmps_engine_opts o{};
+  o.device = E->device;
+  o.hbm_budget_bytes = E->budget;  // admission bounds the projected footprint of both groups together
+  o.jacobi_max_sweeps = E->max_sweeps;
+  o.jacobi_tol = E->tol;
+  o.jacobi_tol_mixed = E->tol_mixed;
+  hipmps_engine B = nullptr;
+  RC_TRY(hipmps_engine_create(&o, &B));
+  hipmps_tuning t{};
+  (void)hipmps_engine_get_tuning(E, &t);
+  t.big_lane_min_chi = 0;
+  t.wave_groups = 1;
+  (void)hipmps_engine_set_tuning(B, &t);
+  B->timing = E->timing;
+  E->grp = B;
+  E->grp_stop = false;
+  E->grp_worker = std::thread(group_loop, E);
+  return 0;
+}
+
+// Start a circuit on the wave group holding less projected footprint among the running ones.
+int start_grouped(hipmps_engine E, PendingCircuit* pc, const std::vector<PendingCircuit*>& running) {
+  pc->grp = 0;
+  if (E->wave_groups >= 2) {
+    RC_TRY(ensure_group_engine(E));
+    size_t load[2] = {0, 0};
+    for (auto* r : running)
+      if (!r->big && !r->finished) load[r->grp] += r->footprint;
+    pc->grp = load[1] < load[0] ? 1 : 0;
+  }
+  return start_circuit(pc->grp ? E->grp : E, pc);
+}
+
+// advance_all over the wave groups: group 1 on the worker, group 0 on the calling thread, joined
+// before returning (each group's circuits are touched by its own thread only).
+int advance_groups(hipmps_engine E, std::vector<PendingCircuit*>& run) {
+  if (!E->grp) return advance_all(E, run);
+  std::vector<PendingCircuit*> g0, g1;
+  for (auto* pc : run) (pc->grp ? g1 : g0).push_back(pc);
+  if (g1.empty()) return advance_all(E, g0);
+  {
+    std::lock_guard<std::mutex> lk(E->grp_mu);
+    E->grp_rc = 0;
+    E->grp_job = &g1;
+  }
+  E->grp_cv.notify_all();
+  const int rc0 = g0.empty() ? 0 : advance_all(E, g0);
+  const std::string err0 = rc0 ? hipmps_last_error() : "";
+  int rc1 = 0;
+  std::string err1;
+  {
+    std::unique_lock<std::mutex> lk(E->grp_mu);
+    E->grp_cv.wait(lk, [&] { return E->grp_job == nullptr; });
+    rc1 = E->grp_rc;
+    err1 = E->grp_err;
+  }
+  if (rc0) return fail(rc0, err0);
+  if (rc1) return fail(rc1, err1);
   return 0;
 }
 
@@ -3136,7 +3283,7 @@ int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits, hip
   while (rc == 0 && (next < queue.size() || !running.empty())) {
     while (next < queue.size() && (running.empty() || inflight + queue[next]->footprint <= e->budget)) {
       PendingCircuit* pc = queue[next++];
-      const int r = start_circuit(e, pc);
+      const int r = start_grouped(e, pc, running);
       if (r == HIPMPS_E_NOMEM) {
         fail_circuit(pc, r);
         continue;
@@ -3149,7 +3296,7 @@ int hipmps_run_batch(hipmps_engine e, int n, const hipmps_circuit* circuits, hip
       inflight += pc->footprint;
     }
     if (rc != 0) break;
-    rc = advance_all(e, running);
+    rc = advance_groups(e, running);
     std::vector<PendingCircuit*> keep;
     for (auto* pc : running) {
       if (pc->finished) {
@@ -3212,7 +3359,7 @@ int hipmps_serve_step(hipmps_engine e, int* n_done) {
       }
       e->big_cv.notify_all();
     } else {
-      const int r = start_circuit(e, pc);
+      const int r = start_grouped(e, pc, e->running);
       if (r == HIPMPS_E_NOMEM) fail_circuit(pc, r);
       else RC_TRY(r);
     }
@@ -3226,7 +3373,7 @@ int hipmps_serve_step(hipmps_engine e, int* n_done) {
     else small.push_back(pc);
   }
   if (!small.empty()) {
-    RC_TRY(advance_all(e, small));
+    RC_TRY(advance_groups(e, small));
   } else if (any_big) {  // only background circuits in flight: wait for some progress
     std::unique_lock<std::mutex> lk(e->big_mu);
     e->big_cv.wait_for(lk, std::chrono::milliseconds(20));
