@@ -588,6 +588,8 @@ def main():
                     help="arrival rate of the second serving block (96 brickwork + 1 grid circuit)")
     ap.add_argument("--serve-big-lane", type=int, default=512,
                     help="serving: circuits whose bond can reach this run on the background lane (0 = off)")
+    ap.add_argument("--lane-min-pairs", type=int, default=-1,
+                    help="block pairs from which a wave's Jacobi rounds use several streams (default: engine default)")
     ap.add_argument("--lq-min-rank", type=int, default=-1,
                     help="Jacobi on the double-LQ factor for min(m,n) >= this (0 off; default: engine default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -620,6 +622,8 @@ def main():
     eng = mps.Engine()
     if args.lq_min_rank >= 0:
         eng.set_lq_precondition(args.lq_min_rank)
+    if args.lane_min_pairs >= 0:
+        eng.set_tuning(lane_min_pairs=args.lane_min_pairs)
     eng.set_timing(True)
 
     log(f"batched {args.precision}: {args.batch} x 40-qubit depth-{depth} Haar brickwork, chi_max {CHI}; "
