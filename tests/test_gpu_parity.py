@@ -393,3 +393,19 @@ def test_tuning_roundtrip_and_results(eng):
             assert np.array_equal(run("fp64"), c64) and np.array_equal(run("mixed"), cmx)
     finally:
         eng.set_tuning(**base)
+
+
+def test_autotune_small_shape(eng):
+    """The autotuner times each candidate tuning on the same circuits and leaves the fastest set
+    on the engine (here: wave groups 1 and 2 at a small shape)."""
+    from tnqvm_amd import autotune
+    say("autotune")
+    base = eng.tuning()
+    try:
+        out = autotune.tune(eng, n=12, chi=16, batch=4, warmup=4, steps=2, lanes=(2,), gram32=(5,), lq=(64,),
+                            groups=(1, 2))
+        assert [r["wave_groups"] for r in out["candidates"]] == [1, 2]
+        assert all(r["applies_per_s"] > 0 for r in out["candidates"])
+        assert eng.tuning()["wave_groups"] == out["choice"]["wave_groups"]
+    finally:
+        eng.set_tuning(**base)

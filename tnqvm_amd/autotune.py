@@ -7,6 +7,7 @@ engine.  The candidates cover:
   jacobi_lanes   1, 2, 4 streams for the Jacobi rounds and LQ panels of large waves
   gram32_sweeps  0, 3, 5, 7 leading sweeps with an FP32-MFMA Gram (mixed states only)
   lq_min_rank    0 (no preconditioning) or 64 (double LQ for min(m, n) >= 64)
+  wave_groups    1 or 2 engines whose waves advance concurrently
 
     python -m tnqvm_amd.autotune [--qubits 40] [--chi 256] [--batch 8] [--warmup 12] [--steps 6]
 prints one JSON object: the table of candidates (applies/s, Jacobi sweeps per SVD) and the
@@ -45,21 +46,22 @@ def time_tuning(eng, circuits, n, chi, warmup, steps, precision="mixed"):
 
 
 def tune(eng, n=40, chi=256, batch=8, warmup=12, steps=6, lanes=(1, 2, 4), gram32=(0, 3, 5, 7), lq=(0, 64),
-         precision="mixed", log=None):
+         groups=(1, 2), precision="mixed", log=None):
     depth = warmup + steps
     circuits = [C.haar_brickwork(n, depth, seed=0x7E5A0A00 + k) for k in range(batch)]
     base = eng.tuning()
     rows = []
-    for jl, g, q in itertools.product(lanes, gram32, lq):
-        eng.set_tuning(jacobi_lanes=jl, gram32_sweeps=g, lq_min_rank=q)
+    for jl, g, q, wg in itertools.product(lanes, gram32, lq, groups):
+        eng.set_tuning(jacobi_lanes=jl, gram32_sweeps=g, lq_min_rank=q, wave_groups=wg)
         rate, sweeps = time_tuning(eng, circuits, n, chi, warmup, steps, precision)
-        rows.append({"jacobi_lanes": jl, "gram32_sweeps": g, "lq_min_rank": q, "applies_per_s": round(rate, 2),
-                     "sweeps_per_svd": round(sweeps, 2)})
+        rows.append({"jacobi_lanes": jl, "gram32_sweeps": g, "lq_min_rank": q, "wave_groups": wg,
+                     "applies_per_s": round(rate, 2), "sweeps_per_svd": round(sweeps, 2)})
         if log:
             log(rows[-1])
     best = max(rows, key=lambda r: r["applies_per_s"])
     eng.set_tuning(jacobi_lanes=best["jacobi_lanes"], gram32_sweeps=best["gram32_sweeps"],
-                   lq_min_rank=best["lq_min_rank"], lane_min_pairs=base["lane_min_pairs"])
+                   lq_min_rank=best["lq_min_rank"], wave_groups=best["wave_groups"],
+                   lane_min_pairs=base["lane_min_pairs"])
     return {"workload": {"qubits": n, "chi": chi, "batch": batch, "warmup_layers": warmup, "timed_layers": steps,
                          "precision": precision},
             "candidates": rows, "choice": best}
