@@ -34,6 +34,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -3198,15 +3199,26 @@ int ensure_group_engine(hipmps_engine E) {
   B->timing = E->timing;
   E->grp = B;
   E->grp_stop = false;
-  E->grp_worker = std::thread(group_loop, E);
+  try {
+    E->grp_worker = std::thread(group_loop, E);
+  } catch (const std::exception& ex) {
+    E->grp = nullptr;
+    (void)hipmps_engine_destroy(B);
+    return fail(HIPMPS_E_DEVICE, std::string("wave group worker: ") + ex.what());
+  }
   return 0;
 }
 
 // Start a circuit on the wave group holding less projected footprint among the running ones.
 int start_grouped(hipmps_engine E, PendingCircuit* pc, const std::vector<PendingCircuit*>& running) {
   pc->grp = 0;
+  if (E->wave_groups >= 2 && ensure_group_engine(E) != 0) {
+    // no second engine on this device (streams or memory): run every circuit in one group,
+    // which the tuning then reports
+    (void)hipGetLastError();
+    E->wave_groups = 1;
+  }
   if (E->wave_groups >= 2) {
-    RC_TRY(ensure_group_engine(E));
     size_t load[2] = {0, 0};
     for (auto* r : running)
       if (!r->big && !r->finished) load[r->grp] += r->footprint;
