@@ -311,11 +311,12 @@ typedef struct {
   int jacobi_max_sweeps; /* sweep cap of every Jacobi SVD (1..1000), after which the SVD counts as
                             non-converged (HIPMPS_E_CONVERGENCE, hipmps_stats_t.svds_nonconverged);
                             default: hipmps_engine_opts.jacobi_max_sweeps */
-  int wave_groups;      /* batches and serving: circuits are split over this many groups (1 or 2,
-                           default 2), each with an engine of its own (streams, arena); the groups'
-                           waves advance concurrently, the second on a worker thread, so that one
-                           group's LQ panels overlap the other's Jacobi rounds.  A circuit's results
-                           do not depend on its group.  hipmps_engine_profile sums both groups */
+  int wave_groups;      /* batches and serving: circuits whose bond can reach 128 are balanced over
+                           this many groups (1 or 2, default 2; smaller circuits run in the first),
+                           each with an engine of its own (streams, arena); the groups' waves advance
+                           concurrently, the second on a worker thread, so that one group's LQ
+                           panels overlap the other's Jacobi rounds.  A circuit's results do not
+                           depend on its group.  hipmps_engine_profile sums both groups */
 } hipmps_tuning;
 int hipmps_engine_get_tuning(hipmps_engine e, hipmps_tuning* out);
 int hipmps_engine_set_tuning(hipmps_engine e, const hipmps_tuning* t);

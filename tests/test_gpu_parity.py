@@ -387,10 +387,28 @@ def test_tuning_roundtrip_and_results(eng):
         c64, cmx = run("fp64"), run("mixed")
         assert np.array_equal(a64, c64)  # FP64 states never take the FP32 Gram
         assert np.abs(cmx - amx).max() < 1e-6
+        # wave groups: circuits whose bond can reach 128 are balanced over the two groups, the
+        # small ones stay in the first
+        heavy = [Cc.haar_brickwork(16, 10, seed=0x7E5A0B10 + k) for k in range(4)]
+
+        def run_mix(precision):
+            jobs = [{"n": 12, "gates": c, "trunc": mps.trunc_opts(1e-10, 32, precision=precision), "zz": []}
+                    for c in circs]
+            jobs += [{"n": 16, "gates": c, "trunc": mps.trunc_opts(1e-10, 128, precision=precision), "zz": []}
+                     for c in heavy]
+            return np.array([r["expval_z"] for r in eng.run_batch(jobs)], dtype=object)
+
+        ref = {}
         for groups in (1, 2):
             eng.set_tuning(wave_groups=groups)
             assert eng.tuning()["wave_groups"] == groups
             assert np.array_equal(run("fp64"), c64) and np.array_equal(run("mixed"), cmx)
+            for p in ("fp64", "mixed"):
+                z = run_mix(p)
+                if groups == 1:
+                    ref[p] = z
+                else:
+                    assert all(np.array_equal(a, b) for a, b in zip(z, ref[p]))
     finally:
         eng.set_tuning(**base)
 

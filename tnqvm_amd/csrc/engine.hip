@@ -3210,6 +3210,7 @@ int ensure_group_engine(hipmps_engine E) {
 }
 
 // Start a circuit on the wave group holding less projected footprint among the running ones.
+constexpr int GROUP_MIN_BOND = 128;
 int start_grouped(hipmps_engine E, PendingCircuit* pc, const std::vector<PendingCircuit*>& running) {
   pc->grp = 0;
   if (E->wave_groups >= 2 && ensure_group_engine(E) != 0) {
@@ -3218,7 +3219,10 @@ int start_grouped(hipmps_engine E, PendingCircuit* pc, const std::vector<Pending
     (void)hipGetLastError();
     E->wave_groups = 1;
   }
-  if (E->wave_groups >= 2) {
+  // only circuits whose bond can reach GROUP_MIN_BOND go to the second group: their SVDs have
+  // LQ panel chains long enough for the other group's Jacobi rounds to fill; the waves of small
+  // circuits are launch-bound, and splitting them only adds a join to every step
+  if (E->wave_groups >= 2 && projected_max_bond(pc->c) >= GROUP_MIN_BOND) {
     size_t load[2] = {0, 0};
     for (auto* r : running)
       if (!r->big && !r->finished) load[r->grp] += r->footprint;
