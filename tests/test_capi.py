@@ -112,3 +112,40 @@ def test_gate_tables_closed_forms(table):
     assert np.allclose(gm("Swap"), swap, atol=0)
     # iSwap = exp(+i pi/4 (XX + YY))
     assert np.allclose(gm("iSwap"), _expm(0.25j * np.pi * (np.kron(X, X) + np.kron(Y, Y))), atol=1e-14)
+
+
+# The ctypes mirrors of the header's structs, checked field by field against what a C compiler
+# makes of include/hipmps.h: size, and each field's offset and size.  A field added to one side
+# only, or reordered, fails here before any call could read the wrong bytes.
+_STRUCTS = [(C.EngineOpts, "hipmps_engine_opts"), (C.TruncOpts, "hipmps_trunc_opts"), (C.Gate, "hipmps_gate"),
+            (C.Tuning, "hipmps_tuning"), (C.Stats, "hipmps_stats_t"), (C.Circuit, "hipmps_circuit"),
+            (C.Result, "hipmps_result"), (C.Profile, "hipmps_profile_t")]
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include "hipmps.h"', "int main(void) {"]
+    for cls, cname in _STRUCTS:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {f} %zu %zu\\n", offsetof({cname}, {f}), sizeof((({cname}*)0)->{f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            parts = ln.split()
+            got[(parts[0], parts[1])] = tuple(int(x) for x in parts[2:])
+    for cls, cname in _STRUCTS:
+        assert got[(cname, "size")] == (ctypes.sizeof(cls),), cname
+        for f, _ in cls._fields_:
+            d = getattr(cls, f)
+            assert got[(cname, f)] == (d.offset, d.size), (cname, f)
