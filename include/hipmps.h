@@ -285,6 +285,8 @@ typedef struct {
                                     index 9 counts 512 <= min(m, n) < 1024, i.e. theta' 512 x 512
                                     when chi_max = 256 */
 } hipmps_profile_t;
+/* with two wave groups (hipmps_tuning.wave_groups) every field is the sum over both groups'
+ * engines, whose phase times overlap in time; the background lane's engine is not included */
 int hipmps_engine_profile(hipmps_engine e, hipmps_profile_t* out, int reset);
 int hipmps_engine_set_timing(hipmps_engine e, int enabled);
 /* Preconditioning of the SVD: for min(m, n) >= min_rank the one-sided Jacobi runs on R2 of the
