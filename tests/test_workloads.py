@@ -240,3 +240,16 @@ def test_reference_gate_brickwork_equals_unitary_brickwork():
     assert abs(abs(np.vdot(a.psi, b.psi)) - 1) < 1e-12
     for q in range(n):
         assert abs(a.expval_z_parity([q]) - b.expval_z_parity([q])) < 1e-12
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: asked for more, it exits with code 2 and says why, before it
+    touches a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
